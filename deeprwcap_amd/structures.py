@@ -1,0 +1,92 @@
+"""Seeded synthetic structure descriptions for the transition query.
+
+A query's dielectric cube is described the way the reference library takes
+it (inference_cpp/src/dnn_solver.h:27-30, dnn_solver.cpp:264-272):
+
+* ``layers``  float32 [B, 15, 2]  = (min_z, eps) per planar z-layer, sorted by
+  min_z, padded with -9999 (first padded min_z ends the list);
+* ``cuboids`` float32 [B, 15, 7]  = (minx, miny, minz, maxx, maxy, maxz, eps),
+  first match wins, padded with -9999;
+
+all in normalised cube coordinates [-1, 1].
+
+The generators below produce the workloads used for parity tests and the
+benchmark:
+
+* :func:`layered_boxes` - 1-3 planar layers whose normal is drawn from
+  {x, y, z} (x/y-normal slabs are expressed as full-span cuboids), interface
+  planes uniform on the grid faces of the query's N, eps ~ U[1, 8], plus
+  ``0..max_boxes`` axis-aligned box inclusions with eps ~ U[1, 8] whose corners
+  lie on grid faces.  All eps are float32 values, so a float64 solve sees
+  exactly the same permittivities.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+PAD = np.float32(-9999.0)
+MAX_LAYERS = 15
+MAX_CUBOIDS = 15
+SPAN = 1.5  # full-span slabs/boxes extend past the cube faces
+
+
+def empty(B: int):
+    layers = np.full((B, MAX_LAYERS, 2), PAD, dtype=np.float32)
+    cuboids = np.full((B, MAX_CUBOIDS, 7), PAD, dtype=np.float32)
+    return layers, cuboids
+
+
+def _grid_coord(p, N):
+    """Coordinate of grid face p (0..N) of an N-cell cube."""
+    return np.float32(-1.0 + 2.0 * p / N)
+
+
+def layered_boxes(B: int, N, seed: int = 0, max_boxes: int = 2, eps_lo: float = 1.0, eps_hi: float = 8.0,
+                  normals=(0, 1, 2)):
+    """Generate B structure descriptions.
+
+    ``N`` is an int or an int array [B] (the resolution each query is solved
+    at; interfaces are put on that grid).  Returns (layers, cuboids, N[B])."""
+    rng = np.random.default_rng(seed)
+    Ns = np.broadcast_to(np.asarray(N, dtype=np.int32), (B,)).copy()
+    layers, cuboids = empty(B)
+    for b in range(B):
+        n = int(Ns[b])
+        nl = int(rng.integers(1, 4))
+        normal = int(normals[int(rng.integers(0, len(normals)))])
+        ifaces = np.sort(rng.choice(np.arange(1, n), size=nl - 1, replace=False)) if nl > 1 else np.zeros(0, int)
+        eps = rng.uniform(eps_lo, eps_hi, size=nl).astype(np.float32)
+        nb = int(rng.integers(0, max_boxes + 1))
+        ci = 0
+        for _ in range(nb):
+            lo = rng.integers(0, n, size=3)
+            hi = np.array([rng.integers(lo[d] + 1, n + 1) for d in range(3)])
+            e = np.float32(rng.uniform(eps_lo, eps_hi))
+            # inclusive box bounds on grid faces: contains cell centres lo..hi-1
+            cuboids[b, ci, 0:3] = [_grid_coord(lo[d], n) for d in range(3)]
+            cuboids[b, ci, 3:6] = [_grid_coord(hi[d], n) for d in range(3)]
+            cuboids[b, ci, 6] = e
+            ci += 1
+        if normal == 2:
+            layers[b, 0] = (-SPAN, eps[0])
+            for l in range(1, nl):
+                layers[b, l] = (_grid_coord(ifaces[l - 1], n), eps[l])
+        else:
+            # background = first slab; later slabs as full-span cuboids
+            layers[b, 0] = (-SPAN, eps[0])
+            for l in range(1, nl):
+                lo = _grid_coord(ifaces[l - 1], n)
+                hi = _grid_coord(ifaces[l], n) if l < nl - 1 else np.float32(SPAN)
+                box = [-SPAN, -SPAN, -SPAN, SPAN, SPAN, SPAN]
+                box[normal] = lo
+                box[normal + 3] = hi
+                cuboids[b, ci, 0:6] = box
+                cuboids[b, ci, 6] = eps[l]
+                ci += 1
+    return layers, cuboids, Ns
+
+
+def mixed_N(B: int, seed: int = 0, choices=(16, 21, 23)):
+    """N drawn uniformly from ``choices`` (deterministic in seed)."""
+    rng = np.random.default_rng(seed ^ 0x5EED)
+    return rng.choice(np.asarray(choices, dtype=np.int32), size=B)
