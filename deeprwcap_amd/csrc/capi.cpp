@@ -1,0 +1,299 @@
+// extern "C" boundary of libdnnsolver (declared in include/dnnsolver_c.h).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dnnsolver_c.h"
+#include "common.h"
+#include "fdm.h"
+#include "kernels.h"
+#include "query.h"
+#include "runtime.h"
+
+using namespace drw;
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t want)
+    {
+        if (want <= n) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        const size_t cap = want + want / 4 + 16;
+        DRW_HIP(hipMalloc(&p, sizeof(T) * cap));
+        n = cap;
+        return 0;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct drw_solver {
+    int kind = DRW_POISSON;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int backend = -1;  // none until load_models / use_fdm
+    int fdm_N = 23;
+    double fdm_tol = 1e-6;
+    uint64_t seed = 0x5eed5eedULL;
+    uint64_t counter = 0;
+    drw::NetWeights* nets = nullptr;
+    DevBuf<float> layers, cuboids, out, eps, cen;
+    DevBuf<int32_t> axis, Nq, code;
+    DevBuf<int64_t> eps_off, G_off;
+    DevBuf<double> G;
+    DevBuf<uint8_t> scratch;
+};
+
+extern "C" {
+
+const char* drw_last_error(void) { return drw::last_error(); }
+int drw_abi_version(void) { return 1; }
+
+int drw_fdm_solve(int B, const int32_t* N, const int32_t* code, const float* eps, const int64_t* eps_off, double* G,
+                  const int64_t* G_off, const int32_t* order, double tol, int32_t* iters, float* relres, void* stream)
+{
+    clear_error();
+    if (B < 0) return set_error(DRW_EINVAL, "drw_fdm_solve: B < 0");
+    if (B == 0) return 0;
+    if (!N || !code || !eps || !eps_off || !G || !G_off) return set_error(DRW_EINVAL, "drw_fdm_solve: null pointer");
+    if (!(tol > 0)) return set_error(DRW_EINVAL, "drw_fdm_solve: tol must be > 0");
+    Workspace* ws = nullptr;
+    int rc = get_workspace((hipStream_t)stream, &ws);
+    if (rc) return rc;
+    FdmBatch a{};
+    a.B = B;
+    a.N = N;
+    a.code = code;
+    a.eps = eps;
+    a.eps_off = eps_off;
+    a.G = G;
+    a.G_off = G_off;
+    a.order = order;
+    a.tol = tol;
+    a.inner_floor = 1e-5f;
+    a.max_inner = 2000;
+    a.max_rounds = 12;
+    a.iters = iters;
+    a.relres = relres;
+    a.y_ws = ws->y_ws;
+    a.s_ws = ws->s_ws;
+    a.counters = ws->counters;
+    return fdm_launch(a, ws->fdm_grid, (hipStream_t)stream);
+}
+
+int drw_voxelize(int B, const float* layers, const float* cuboids, const int32_t* N, int fixed_N, int normalise,
+                 float* out, const int64_t* out_off, float* maxv, void* stream)
+{
+    clear_error();
+    if (B < 0) return set_error(DRW_EINVAL, "drw_voxelize: B < 0");
+    if (!layers || !cuboids || !out) return set_error(DRW_EINVAL, "drw_voxelize: null pointer");
+    if (!N && (fixed_N < 1 || fixed_N > 64)) return set_error(DRW_EINVAL, "drw_voxelize: fixed_N out of range");
+    VoxelArgs a{B, layers, cuboids, N, fixed_N, normalise, out, out_off, maxv};
+    return voxelize_launch(a, (hipStream_t)stream);
+}
+
+int drw_sample_flat(int B, const int32_t* N, const int32_t* axis, const double* G, const int64_t* G_off,
+                    const float* center_eps, uint64_t seed, uint64_t counter, float* out, void* stream)
+{
+    clear_error();
+    if (B < 0) return set_error(DRW_EINVAL, "drw_sample_flat: B < 0");
+    if (!N || !G || !G_off || !out) return set_error(DRW_EINVAL, "drw_sample_flat: null pointer");
+    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out};
+    return sample_flat_launch(a, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ solvers
+int drw_solver_create(int kind, int device_idx, drw_solver** out)
+{
+    clear_error();
+    if (!out) return set_error(DRW_EINVAL, "drw_solver_create: out is null");
+    *out = nullptr;
+    if (kind != DRW_POISSON && kind != DRW_GRADIENT) return set_error(DRW_EINVAL, "drw_solver_create: bad kind %d", kind);
+    int ndev = 0;
+    DRW_HIP(hipGetDeviceCount(&ndev));
+    if (device_idx < 0 || device_idx >= ndev)
+        return set_error(DRW_EINVAL, "drw_solver_create: device %d out of range (%d devices)", device_idx, ndev);
+    DRW_HIP(hipSetDevice(device_idx));
+    drw_solver* s = new drw_solver();
+    s->kind = kind;
+    s->device = device_idx;
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete s;
+        return set_error(DRW_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return 0;
+}
+
+void drw_solver_destroy(drw_solver* s)
+{
+    if (!s) return;
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->layers.release(); s->cuboids.release(); s->out.release(); s->eps.release(); s->cen.release();
+    s->axis.release(); s->Nq.release(); s->code.release(); s->eps_off.release(); s->G_off.release();
+    s->G.release(); s->scratch.release();
+    if (s->nets) net_weights_free(s->nets);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+int drw_solver_load_models(drw_solver* s, const char* models_txt)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "drw_solver_load_models: null solver");
+    DRW_HIP(hipSetDevice(s->device));
+    NetWeights* w = nullptr;
+    int rc = net_weights_load(models_txt ? models_txt : "models.txt", s->kind, &w);
+    if (rc) return rc;
+    if (s->nets) net_weights_free(s->nets);
+    s->nets = w;
+    s->backend = DRW_BACKEND_NN;
+    return 0;
+}
+
+int drw_solver_use_fdm(drw_solver* s, int N, double tol)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "drw_solver_use_fdm: null solver");
+    if (!fdm_supported_n(N)) return set_error(DRW_EINVAL, "drw_solver_use_fdm: N=%d not supported", N);
+    if (!(tol > 0)) return set_error(DRW_EINVAL, "drw_solver_use_fdm: tol must be > 0");
+    s->fdm_N = N;
+    s->fdm_tol = tol;
+    s->backend = DRW_BACKEND_FDM;
+    return 0;
+}
+
+int drw_solver_backend(const drw_solver* s) { return s ? s->backend : -1; }
+
+int drw_solver_set_seed(drw_solver* s, uint64_t seed, uint64_t counter)
+{
+    if (!s) return set_error(DRW_EINVAL, "drw_solver_set_seed: null solver");
+    s->seed = seed;
+    s->counter = counter;
+    return 0;
+}
+
+int drw_solver_get_counter(const drw_solver* s, uint64_t* counter)
+{
+    if (!s || !counter) return set_error(DRW_EINVAL, "drw_solver_get_counter: null argument");
+    *counter = s->counter;
+    return 0;
+}
+
+int drw_solver_output_width(const drw_solver* s) { return s ? (s->kind == DRW_GRADIENT ? 5 : 3) : -1; }
+
+static int solver_run(drw_solver* s, int B, const float* L, const float* C, const int32_t* ax, float* out,
+                      hipStream_t st)
+{
+    const bool grad = s->kind == DRW_GRADIENT;
+    if (s->backend == DRW_BACKEND_FDM) {
+        const int N = s->fdm_N;
+        int rc;
+        if ((rc = s->eps.ensure((size_t)B * N * N * N)) || (rc = s->G.ensure((size_t)B * 6 * N * N)) ||
+            (rc = s->Nq.ensure(B)) || (rc = s->code.ensure(B)) || (rc = s->eps_off.ensure(B)) ||
+            (rc = s->G_off.ensure(B)) || (rc = s->cen.ensure(B)))
+            return rc;
+        VoxelArgs va{B, L, C, nullptr, N, 0, s->eps.p, nullptr, nullptr};
+        if ((rc = voxelize_launch(va, st))) return rc;
+        if ((rc = fill_meta_launch(B, N, grad ? ax : nullptr, s->Nq.p, s->code.p, s->eps_off.p, s->G_off.p, st)))
+            return rc;
+        Workspace* ws = nullptr;
+        if ((rc = get_workspace(st, &ws))) return rc;
+        FdmBatch a{};
+        a.B = B; a.N = s->Nq.p; a.code = s->code.p; a.eps = s->eps.p; a.eps_off = s->eps_off.p;
+        a.G = s->G.p; a.G_off = s->G_off.p; a.order = nullptr; a.tol = s->fdm_tol; a.inner_floor = 1e-5f;
+        a.max_inner = 2000; a.max_rounds = 12; a.iters = nullptr; a.relres = nullptr;
+        a.y_ws = ws->y_ws; a.s_ws = ws->s_ws; a.counters = ws->counters;
+        if ((rc = fdm_launch(a, ws->fdm_grid, st))) return rc;
+        if (grad && (rc = center_eps_launch(B, s->Nq.p, ax, s->eps.p, s->eps_off.p, s->cen.p, st))) return rc;
+        FlatSampleArgs fa{B, s->Nq.p, grad ? ax : nullptr, s->G.p, s->G_off.p, grad ? s->cen.p : nullptr,
+                          s->seed, s->counter, out};
+        if ((rc = sample_flat_launch(fa, st))) return rc;
+        s->counter += (uint64_t)B;
+        return 0;
+    }
+    if (s->backend == DRW_BACKEND_NN) {
+        QueryArgs q{};
+        q.B = B; q.layers = L; q.cuboids = C; q.axis = grad ? ax : nullptr; q.out = out;
+        q.seed = s->seed; q.counter = s->counter;
+        int rc = query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, st);
+        if (rc) return rc;
+        s->counter += (uint64_t)B;
+        return 0;
+    }
+    return set_error(DRW_ESTATE, "solver has no backend: call drw_solver_load_models or drw_solver_use_fdm");
+}
+
+static void warn_axis_ignored()
+{
+    static std::once_flag once;
+    std::call_once(once, [] {
+        fprintf(stderr,
+                "Warning: Axis parameter provided to Green's function solver but will be ignored. "
+                "(This warning will only be shown once)\n");
+    });
+}
+
+int drw_solver_sample_device(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                             float* out, void* stream)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "sample: null solver");
+    if (B < 0) return set_error(DRW_EINVAL, "sample: B < 0");
+    if (B == 0) return 0;
+    if (!layers || !cuboids || !out) return set_error(DRW_EINVAL, "sample: null pointer");
+    if (s->kind == DRW_GRADIENT && !axis) return set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    if (s->kind == DRW_POISSON && axis) warn_axis_ignored();
+    DRW_HIP(hipSetDevice(s->device));
+    return solver_run(s, B, layers, cuboids, axis, out, stream ? (hipStream_t)stream : s->stream);
+}
+
+int drw_solver_sample(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                      float* out)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "sample: null solver");
+    if (B < 0) return set_error(DRW_EINVAL, "sample: B < 0");
+    if (B == 0) return 0;
+    if (!layers || !cuboids || !out) return set_error(DRW_EINVAL, "sample: null pointer");
+    if (s->kind == DRW_GRADIENT && !axis) return set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    if (s->kind == DRW_POISSON && axis) warn_axis_ignored();
+    if (s->backend < 0)
+        return set_error(DRW_ESTATE, "solver has no backend: call drw_solver_load_models or drw_solver_use_fdm");
+    DRW_HIP(hipSetDevice(s->device));
+    const int width = drw_solver_output_width(s);
+    int rc;
+    if ((rc = s->layers.ensure((size_t)B * kMaxLayers * 2)) || (rc = s->cuboids.ensure((size_t)B * kMaxCuboids * 7)) ||
+        (rc = s->out.ensure((size_t)B * width)) || (rc = s->axis.ensure(B)))
+        return rc;
+    hipStream_t st = s->stream;
+    DRW_HIP(hipMemcpyAsync(s->layers.p, layers, sizeof(float) * B * kMaxLayers * 2, hipMemcpyHostToDevice, st));
+    DRW_HIP(hipMemcpyAsync(s->cuboids.p, cuboids, sizeof(float) * B * kMaxCuboids * 7, hipMemcpyHostToDevice, st));
+    const bool grad = s->kind == DRW_GRADIENT;
+    if (grad) DRW_HIP(hipMemcpyAsync(s->axis.p, axis, sizeof(int32_t) * B, hipMemcpyHostToDevice, st));
+    rc = solver_run(s, B, s->layers.p, s->cuboids.p, grad ? s->axis.p : nullptr, s->out.p, st);
+    if (rc) return rc;
+    DRW_HIP(hipMemcpyAsync(out, s->out.p, sizeof(float) * B * width, hipMemcpyDeviceToHost, st));
+    DRW_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,30 @@
+// Network-backed transition queries (the five predictor networks at N = 23).
+#pragma once
+
+#include "common.h"
+
+namespace drw {
+
+struct NetWeights;  // device-resident packed weights of one solver kind
+
+// Loads the index-addressed model list (models.txt semantics,
+// inference_cpp/src/dnn_solver.cpp:65-87): entries 0,1 for Poisson solvers,
+// 2,3,4 for gradient solvers (dnn_solver_poisson.cpp:18-20,
+// dnn_solver_gradient.cpp:19-22).
+int net_weights_load(const char* models_txt, int kind, NetWeights** out);
+void net_weights_free(NetWeights* w);
+
+struct QueryArgs {
+    int B;
+    const float* layers;    // [B, 15, 2]
+    const float* cuboids;   // [B, 15, 7]
+    const int32_t* axis;    // [B] (gradient) or null (Poisson)
+    float* out;             // [B, 3] or [B, 5]
+    uint64_t seed, counter;
+};
+
+// Enqueues the fused network query; *scratch (grown as needed, *cap bytes)
+// holds per-query intermediates.
+int query_nn_launch(const NetWeights* w, const QueryArgs& q, uint8_t** scratch, size_t* cap, hipStream_t stream);
+
+}  // namespace drw

@@ -1,0 +1,137 @@
+// Device helpers shared by the sampler and the fused query kernels:
+// Philox4x32-10, the wave64 fixed-point inverse-CDF draw, index remaps and
+// the panel-centre table.
+#pragma once
+
+#include "common.h"
+
+namespace drw {
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), published multipliers
+// and Weyl increments.  counter = (q_lo, q_hi, stream, 0), key = seed.
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; r++) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__host__ __device__ inline void query_uniforms(uint64_t seed, uint64_t q, uint32_t stream, uint32_t out[4])
+{
+    out[0] = (uint32_t)q;
+    out[1] = (uint32_t)(q >> 32);
+    out[2] = stream;
+    out[3] = 0u;
+    philox4x32_10(out, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// ---------------------------------------------------------------- wave ops
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src)
+{
+    const uint32_t lo = __shfl((int)(uint32_t)v, src, 64), hi = __shfl((int)(uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d)
+{
+    const uint32_t lo = __shfl_up((int)(uint32_t)v, d, 64), hi = __shfl_up((int)(uint32_t)(v >> 32), d, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quant(float a, int s) { return (uint32_t)ldexpf(fabsf(a), s); }
+__device__ __forceinline__ uint32_t quant(double a, int s) { return (uint32_t)ldexp(fabs(a), s); }
+__device__ __forceinline__ int frexp_exp(float m) { int e; frexpf(m, &e); return e; }
+__device__ __forceinline__ int frexp_exp(double m) { int e; frexp(m, &e); return e; }
+
+// Fixed-point inverse-CDF draw over |w[0..M)| by one wave64 (all lanes call,
+// all lanes get the result).  q_i = trunc(|w_i| 2^s) with the power of two s
+// that puts max|w| in [2^23, 2^24); the prefix sums are exact integers, so the
+// drawn bin (smallest i with cumsum_i > floor(u total / 2^32)) is independent
+// of the summation order and bit-identical to the CPU sampler.  Returns -1 if
+// every weight is zero or the largest is not finite.  w may live in any
+// address space (global or LDS).
+template <typename T, typename Ptr>
+__device__ int wave_cdf_draw(Ptr w, int M, uint32_t u)
+{
+    const int lane = threadIdx.x & 63;
+    T m = 0;
+    for (int i = lane; i < M; i += 64) {
+        const T a = w[i] < 0 ? -w[i] : w[i];
+        m = a > m ? a : m;
+    }
+    m = wave_max(m);
+    if (!(m > 0) || isinf(m)) return -1;
+    const int s = 24 - frexp_exp(m);
+    const int C = (M + 63) / 64;
+    const int i0 = lane * C, i1 = min(i0 + C, M);
+    uint64_t loc = 0;
+    for (int i = i0; i < i1; i++) loc += quant(w[i], s);
+    uint64_t incl = loc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t o = shfl_up_u64(incl, d);
+        if (lane >= d) incl += o;
+    }
+    const uint64_t total = shfl_u64(incl, 63);
+    const uint64_t lo = (uint64_t)u * (total & 0xFFFFFFFFull);
+    const uint64_t hi = (uint64_t)u * (total >> 32);
+    const uint64_t r = hi + (lo >> 32);  // floor(u * total / 2^32), total < 2^48
+    const uint64_t hit = __ballot(incl > r);
+    const int hl = hit ? __ffsll((unsigned long long)hit) - 1 : 63;
+    int idx = M - 1;
+    if (lane == hl) {
+        uint64_t acc = incl - loc;
+        for (int i = i0; i < i1; i++) {
+            acc += quant(w[i], s);
+            if (acc > r) { idx = i; break; }
+        }
+    }
+    return __shfl(idx, hl, 64);
+}
+
+// Panel centre of tile t on face f (locate_index_simple_kernel,
+// inference_cpp/src/cuda_kernels.cu:219-264), for any N.
+__device__ __forceinline__ void panel_point(int N, int t, int f, float* o)
+{
+    const int xid = t % N, yid = t / N;
+    const float t0 = (2.0f * xid + 1.0f) / N - 1.0f, t1 = (2.0f * yid + 1.0f) / N - 1.0f;
+    switch (f) {
+    case 0: o[0] = t0; o[1] = t1; o[2] = -1.0f; break;
+    case 1: o[0] = t0; o[1] = t1; o[2] = 1.0f; break;
+    case 2: o[0] = t0; o[1] = -1.0f; o[2] = t1; break;
+    case 3: o[0] = t0; o[1] = 1.0f; o[2] = t1; break;
+    case 4: o[0] = -1.0f; o[1] = t0; o[2] = t1; break;
+    default: o[0] = 1.0f; o[1] = t0; o[2] = t1; break;
+    }
+}
+
+// Face index in the original frame of face f of the axis-pre-rotated cube
+// (locate_index_kernel, cuda_kernels.cu:161-173).
+__host__ __device__ __forceinline__ int axis_face_remap(int f, int axis)
+{
+    if (axis == 0) return f == 0 ? 4 : f == 1 ? 5 : f == 4 ? 1 : f == 5 ? 0 : f;
+    if (axis == 1) return f == 0 ? 2 : f == 1 ? 3 : f == 2 ? 1 : f == 3 ? 0 : f;
+    return f;
+}
+
+}  // namespace drw

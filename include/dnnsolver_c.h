@@ -1,0 +1,120 @@
+/*
+ * libdnnsolver (MI355X / gfx950) - C ABI of the deepRWCap transition-query path.
+ *
+ * The reference library exposes only C++ classes whose members are LibTorch
+ * types (inference_cpp/src/dnn_solver.h:43-201).  This header is the plain C
+ * boundary those classes are rebuilt on: opaque handles, plain pointers and
+ * sizes, int return codes (0 = success, < 0 = error; the message is in
+ * drw_last_error() of the calling thread) and no exceptions across the ABI.
+ * include/dnn_solver.h restores the reference's class interface on top.
+ *
+ * Data layouts (reference conventions):
+ *   layers  float[B][15][2]  (min_z, eps), padded with -9999   dnn_solver.h:27-30
+ *   cuboids float[B][15][7]  (minx,miny,minz,maxx,maxy,maxz,eps) dnn_solver.cpp:264-272
+ *   axis    int32[B] in {0:X, 1:Y, 2:Z}                         dnn_solver.h:32-36
+ *   Poisson output  float[B][3]  next-hop point                  dnn_solver_poisson.cpp:111
+ *   gradient output float[B][5]  (x, y, z, W*sign, eps_centre)   dnn_solver_gradient.cpp:268-273
+ *   cubes  float, [z][y][x] (x fastest); FDM kernels G double[6 N^2] in GGFT
+ *   panel order (faces z-, z+, y-, y+, x-, x+; ggft/src/ggft_solver.h:65-77).
+ *
+ * Functions taking a `stream` argument take device pointers and enqueue on
+ * that hipStream_t (NULL = the null stream) without synchronising.
+ */
+#ifndef DNNSOLVER_C_H
+#define DNNSOLVER_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DRW_OK 0
+#define DRW_EINVAL (-2)   /* bad argument */
+#define DRW_EHIP (-3)     /* HIP runtime error */
+#define DRW_ESTATE (-4)   /* object not ready (e.g. models not loaded) */
+#define DRW_EIO (-5)      /* file missing or malformed */
+
+enum drw_kind { DRW_POISSON = 0, DRW_GRADIENT = 1 };
+enum drw_backend { DRW_BACKEND_NN = 0, DRW_BACKEND_FDM = 1 };
+enum drw_code { DRW_GFT = 0, DRW_WVTZ = 1, DRW_WVTX = 2, DRW_WVTY = 3 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* drw_last_error(void);
+int drw_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Batched FDM surface Green's function (the GGFT judge / fallback).
+ * Replaces ggft_solver::solve (ggft/src/ggft_solver.cpp:521-636) for a batch:
+ * per cube b of resolution N[b] in {8, 9, 16, 21, 23}, right-hand side
+ * code[b] (DRW_GFT/WVTZ/WVTX/WVTY), writes G = 2 eps y on the 6 N^2 panels
+ * (the reference's gft = A13 x).  FP32 Jacobi-PCG with FP64 residual
+ * refinement until ||e - L y|| <= tol ||e||; tol <= 1e-12 gives FP64-accurate
+ * kernels.  order (optional) is the processing order of the cubes; iters /
+ * relres (optional) receive the FP32 iteration count (-1: unsupported N) and
+ * the final FP64 relative residual of each cube.
+ * ------------------------------------------------------------------------ */
+int drw_fdm_solve(int B, const int32_t* N, const int32_t* code, const float* eps, const int64_t* eps_off,
+                  double* G, const int64_t* G_off, const int32_t* order, double tol, int32_t* iters, float* relres,
+                  void* stream);
+
+/* Voxelize the structure description (fill_structure_kernel,
+ * inference_cpp/src/cuda_kernels.cu:21-84, with the per-query eps/max
+ * normalisation of dnn_solver.cpp:274-285 when normalise != 0).  N (optional)
+ * gives each query's resolution, else fixed_N; out_off (optional) the float
+ * offset of each cube, else b*N^3; maxv (optional) receives the per-query max. */
+int drw_voxelize(int B, const float* layers, const float* cuboids, const int32_t* N, int fixed_N, int normalise,
+                 float* out, const int64_t* out_off, float* maxv, void* stream);
+
+/* Flat inverse-CDF next hop over each query's FDM kernel (ggft_solver.cpp:
+ * 638-685 semantics) with Philox4x32-10 uniforms (key seed, counter
+ * counter + b).  axis == NULL: Poisson output [B][3]; else gradient output
+ * [B][5] with W = sum|G| and eps_centre from center_eps (optional). */
+int drw_sample_flat(int B, const int32_t* N, const int32_t* axis, const double* G, const int64_t* G_off,
+                    const float* center_eps, uint64_t seed, uint64_t counter, float* out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Solver objects: DNNSolverPoisson / DNNSolverGrad (dnn_solver.h:98-149).
+ * ------------------------------------------------------------------------ */
+typedef struct drw_solver drw_solver;
+
+/* DNNSolver(int device_idx) ctor, dnn_solver.cpp:13-38.  The solver owns a
+ * HIP stream and its device buffers.  Not thread-safe (like the reference). */
+int drw_solver_create(int kind, int device_idx, drw_solver** out);
+void drw_solver_destroy(drw_solver* s);
+
+/* loadModels(), dnn_solver_poisson.cpp:11-33 / dnn_solver_gradient.cpp:11-38:
+ * reads the index-addressed model list (models.txt semantics,
+ * dnn_solver.cpp:65-87; NULL = "models.txt" in the working directory) and
+ * selects the network backend. */
+int drw_solver_load_models(drw_solver* s, const char* models_txt);
+
+/* Select the FDM backend (the plain batched-FDM fallback and judge): queries
+ * are voxelized at resolution N and solved to relative residual tol. */
+int drw_solver_use_fdm(drw_solver* s, int N, double tol);
+int drw_solver_backend(const drw_solver* s);
+
+/* Philox key and the next query counter (each sampled query consumes one). */
+int drw_solver_set_seed(drw_solver* s, uint64_t seed, uint64_t counter);
+int drw_solver_get_counter(const drw_solver* s, uint64_t* counter);
+
+/* Output floats per query: 3 (Poisson) or 5 (gradient). */
+int drw_solver_output_width(const drw_solver* s);
+
+/* sample(layers, cuboids[, axis]) with host pointers; blocks until `out`
+ * holds B * width floats.  B = size(layers) / 30 in the reference
+ * (dnn_solver_poisson.cpp:40).  Gradient solvers require axis (the reference
+ * throws, dnn_solver_gradient.cpp:40-47); Poisson solvers ignore it. */
+int drw_solver_sample(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                      float* out);
+
+/* Same with device pointers, enqueued on `stream` (NULL: the solver's own). */
+int drw_solver_sample_device(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                             float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DNNSOLVER_C_H */
