@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Transition-query benchmark on one MI355X.
+
+A step is one batch of N = 23 Poisson transition queries (structure
+descriptions already resident in HBM) through the fused network path:
+voxelize-on-the-fly -> PoissonSelector -> face draw -> PoissonPredictor on the
+rotated cube -> tile draw -> locate (inference_cpp/src/dnn_solver_poisson.cpp:
+35-111).  The same batch is also run through the batched FP32-PCG/FP64-
+refinement FDM path (the 20x denominator), the networks' full kernels are
+compared with FP64 FDM kernels (L1 gate), small-batch latency is sampled,
+and the CPU path (the FP32 PyTorch restatement of the networks) is timed on
+the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PRED_FLOP = 2 * 646_438      # PoissonPredictor MACs per face evaluation (architecture.py:44-72)
+SEL_FLOP = 2 * 287_972       # PoissonSelector MACs per query (architecture.py:176-205)
+HBM_PEAK_GBS = 8000.0
+FP16_PEAK_TFLOPS = 2500.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def models_txt_or_random(path):
+    from deeprwcap_amd import weights as W
+    if path and os.path.exists(path):
+        return path, "trained"
+    d = tempfile.mkdtemp(prefix="drw_models_")
+    return W.export_model_dir(d, {n: W.random_state(n, i) for i, n in enumerate(W.MODEL_ORDER)}), "random"
+
+
+def stage_times(solver):
+    import ctypes
+    from deeprwcap_amd import _lib
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_int64 * 8)()
+    _lib.check(_lib.lib().drw_solver_stage_times(solver.handle, ms, cnt, 1), "stage_times")
+    return list(ms), list(cnt)
+
+
+def time_steps(run, steps, warmup, torch):
+    for _ in range(warmup):
+        run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def l1_vs_fdm(solver, L, C, n, torch):
+    """Full-kernel L1 of the networks against FP64 FDM kernels (N = 23)."""
+    from deeprwcap_amd import _lib, device as D
+    dL, dC = torch.from_numpy(L[:n]).cuda(), torch.from_numpy(C[:n]).cuda()
+    probs = torch.empty((n, 6), device="cuda")
+    wts = torch.empty(n, device="cuda")
+    kern = torch.empty((n, 6, 529), device="cuda")
+    _lib.check(_lib.lib().drw_solver_eval_kernels(solver.handle, n, dL.data_ptr(), dC.data_ptr(), None,
+                                                   probs.data_ptr(), wts.data_ptr(), kern.data_ptr(), None),
+               "eval_kernels")
+    pk = D.PackedCubes([23] * n)
+    D.voxelize(L[:n], C[:n], fixed_N=23, normalise=False, out=pk.eps.view(n, 23, 23, 23))
+    D.fdm_solve(pk, 0, tol=1e-10, want_stats=False)
+    torch.cuda.synchronize()
+    G = pk.G.view(n, 6, 529).cpu().numpy()
+    Gn = probs.cpu().numpy()[:, :, None] * kern.cpu().numpy()
+    l1 = np.abs(Gn - G).sum(axis=(1, 2)) / np.abs(G).sum(axis=(1, 2))
+    return float(l1.mean()), float(np.percentile(l1, 99))
+
+
+def cpu_baseline(L, C, models_txt, seconds=15.0):
+    """FP32 PyTorch restatement of the network query on all host cores."""
+    import torch
+    from deeprwcap_amd import weights as W
+    from oracle import nets_ref as R
+    from oracle import oracle as O
+    cores = os.cpu_count() or 1
+    torch.set_num_threads(cores)
+    base = os.path.dirname(models_txt)
+    paths = [l.split("#")[0].strip() for l in open(models_txt) if l.strip() and not l.startswith("#")]
+    P = {n: {k: torch.from_numpy(v) for k, v in W.read_drwnet(os.path.join(base, p)).items()}
+         for n, p in zip(W.MODEL_ORDER, paths)}
+    B = 256
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        sl = slice(done % (len(L) - B), done % (len(L) - B) + B)
+        cubes, _ = O.voxelize(L[sl], C[sl], 23, normalise=True)
+        with torch.no_grad():
+            x = torch.from_numpy(cubes)[:, None, None]
+            probs = R.forward("PoissonSelector", x, P["PoissonSelector"]).numpy()
+            faces = np.array([O.cdf_draw(probs[b], int(O.query_uniforms(1, done + b)[0])) for b in range(B)])
+            rot = np.stack([O.rotate_poisson(cubes[b], int(faces[b])) for b in range(B)])
+            k = R.forward("PoissonPredictor", torch.from_numpy(rot)[:, None, None], P["PoissonPredictor"])
+            k = k.reshape(B, -1).numpy()
+        for b in range(B):
+            t = O.cdf_draw(k[b], int(O.query_uniforms(1, done + b)[1]))
+            O.locate(23, t, int(faces[b]))
+        done += B
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"{done} N=23 Poisson queries (voxelize, PoissonSelector, face draw, rotated "
+                      f"PoissonPredictor, tile draw, locate) in FP32 PyTorch on {cores} threads, {dt:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--models", default=os.path.join(ROOT, "models", "models.txt"))
+    ap.add_argument("--fdm-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the query path has no cross-GPU exchange; see DESIGN.md for how it shards)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from deeprwcap_amd import _lib, structures as S
+    from deeprwcap_amd.dnnsolver import DNNSolverPoisson
+
+    torch.cuda.set_device(0)
+    B = args.batch
+    L, C, _ = S.layered_boxes(B, 23, seed=0)
+    dL, dC = torch.from_numpy(L).cuda(), torch.from_numpy(C).cuda()
+    out = torch.empty((B, 3), device="cuda")
+    stream = torch.cuda.current_stream()
+    models_txt, wkind = models_txt_or_random(args.models)
+    log(f"weights: {wkind} ({models_txt})")
+
+    nn = DNNSolverPoisson(0)
+    nn.loadModels(models_txt=models_txt)
+    nn.set_seed(12345, 0)
+    _lib.check(_lib.lib().drw_solver_set_timing(nn.handle, 1), "set_timing")
+
+    def run_nn():
+        nn.sample_device(B, dL, dC, None, out, stream=stream)
+
+    run_nn()
+    torch.cuda.synchronize()
+    stage_times(nn)  # drop the cold launch
+    dt = time_steps(run_nn, args.steps, args.warmup, torch)
+    ms, cnt = stage_times(nn)
+    value = B * args.steps / dt
+    face_ms = ms[2] / max(cnt[2], 1)
+    sel_ms = (ms[0] + ms[1]) / max(cnt[0], 1)
+    log(f"NN: {value:,.0f} queries/s, {1000 * dt / args.steps:.3f} ms/step; face kernel {face_ms:.3f} ms, "
+        f"selector {sel_ms:.3f} ms per launch")
+
+    # FDM path on the same batch (the batched FP32-FDM query rate)
+    fdm = DNNSolverPoisson(0)
+    fdm.use_fdm(23, 1e-6)
+    _lib.check(_lib.lib().drw_solver_set_timing(fdm.handle, 1), "set_timing")
+
+    def run_fdm():
+        fdm.sample_device(B, dL, dC, None, out, stream=stream)
+
+    dt_f = time_steps(run_fdm, args.fdm_steps, 1, torch)
+    fdm_rate = B * args.fdm_steps / dt_f
+    fms, fcnt = stage_times(fdm)
+    log(f"FDM: {fdm_rate:,.0f} queries/s (FDM kernel {fms[4] / max(fcnt[4], 1):.3f} ms per launch)")
+
+    # accuracy gate: network full kernels vs FP64 FDM
+    l1_mean, l1_p99 = l1_vs_fdm(nn, L, C, 256, torch)
+    log(f"L1 vs FP64 FDM: mean {l1_mean:.4f} p99 {l1_p99:.4f}")
+
+    # small-batch latency (one synchronous batch of 64 queries)
+    lat = []
+    for i in range(200):
+        t0 = time.perf_counter()
+        nn.sample_device(64, dL, dC, None, out, stream=stream)
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat[20:]) * 1e3
+
+    achieved = B * PRED_FLOP / (face_ms * 1e-3) / 1e12
+    res = {
+        "metric": "transition queries/s at N=23 (bf16 NN vs FP32 FDM, <=1e-2 L1); p99 latency",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp16",
+        "data": "synthetic",
+        "config": {"workload": f"poisson_transition_queries_n23_batch{B}", "global_batch": B, "N": 23,
+                   "structures": "1-3 planar layers (normal x|y|z) + 0-2 boxes, eps U[1,8], seed 0",
+                   "weights": wkind, "parallelism": "single_gpu"},
+        "fdm_queries_per_s": fdm_rate,
+        "nn_over_fdm": value / fdm_rate,
+        "l1_vs_fp64_mean": l1_mean,
+        "l1_vs_fp64_p99": l1_p99,
+        "latency_ms_batch64": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "stage_ms_per_launch": {"selector_backbone": ms[0] / max(cnt[0], 1), "selector_head": ms[1] / max(cnt[1], 1),
+                                "face_predictor": face_ms},
+        "roofline": {"kernel": "face_query_kernel<0> (PoissonPredictor)", "bound": "mfma", "achieved": achieved,
+                     "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP16_PEAK_TFLOPS,
+                     "traffic": None,
+                     "note": "algorithmic 1,292,876 FLOP per query; most of it is depthwise conv + GELU on the VALU"},
+    }
+    if not args.no_cpu:
+        log("timing the CPU path ...")
+        res["cpu_baseline"] = cpu_baseline(L, C, models_txt)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -31,8 +31,13 @@ SIGNATURES = {
     "drw_solver_set_seed": (_c_int, [_c_vp, _c_u64, _c_u64]),
     "drw_solver_get_counter": (_c_int, [_c_vp, ctypes.POINTER(_c_u64)]),
     "drw_solver_output_width": (_c_int, [_c_vp]),
+    "drw_solver_set_timing": (_c_int, [_c_vp, _c_int]),
+    "drw_solver_stage_times": (_c_int, [_c_vp, _c_vp, _c_vp, _c_int]),
     "drw_solver_sample": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_device": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "drw_solver_sample_ex": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
+                                      _c_vp]),
+    "drw_solver_eval_kernels": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
 }
 
 _LIB = None

@@ -58,6 +58,7 @@ struct drw_solver {
     DevBuf<int64_t> eps_off, G_off;
     DevBuf<double> G;
     DevBuf<uint8_t> scratch;
+    StageTimer timer;
 };
 
 extern "C" {
@@ -151,6 +152,7 @@ void drw_solver_destroy(drw_solver* s)
     s->axis.release(); s->Nq.release(); s->code.release(); s->eps_off.release(); s->G_off.release();
     s->G.release(); s->scratch.release();
     if (s->nets) net_weights_free(s->nets);
+    s->timer.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -200,8 +202,37 @@ int drw_solver_get_counter(const drw_solver* s, uint64_t* counter)
 
 int drw_solver_output_width(const drw_solver* s) { return s ? (s->kind == DRW_GRADIENT ? 5 : 3) : -1; }
 
+int drw_solver_set_timing(drw_solver* s, int enable)
+{
+    if (!s) return set_error(DRW_EINVAL, "set_timing: null solver");
+    s->timer.enabled = enable != 0;
+    return 0;
+}
+
+int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int reset)
+{
+    clear_error();
+    if (!s || !total_ms || !counts) return set_error(DRW_EINVAL, "stage_times: null argument");
+    DRW_HIP(hipSetDevice(s->device));
+    int rc = s->timer.collect();
+    if (rc) return rc;
+    for (int i = 0; i < StageTimer::kStages; i++) {
+        total_ms[i] = s->timer.total_ms[i];
+        counts[i] = s->timer.count[i];
+        if (reset) { s->timer.total_ms[i] = 0; s->timer.count[i] = 0; }
+    }
+    return 0;
+}
+
+struct Diag {
+    int32_t* face = nullptr;
+    float* probs = nullptr;
+    float* weight = nullptr;
+    float* kern = nullptr;
+};
+
 static int solver_run(drw_solver* s, int B, const float* L, const float* C, const int32_t* ax, float* out,
-                      hipStream_t st)
+                      hipStream_t st, const Diag& dg = Diag())
 {
     const bool grad = s->kind == DRW_GRADIENT;
     if (s->backend == DRW_BACKEND_FDM) {
@@ -211,8 +242,9 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
             (rc = s->Nq.ensure(B)) || (rc = s->code.ensure(B)) || (rc = s->eps_off.ensure(B)) ||
             (rc = s->G_off.ensure(B)) || (rc = s->cen.ensure(B)))
             return rc;
+        StageTimer* tm = &s->timer;
         VoxelArgs va{B, L, C, nullptr, N, 0, s->eps.p, nullptr, nullptr};
-        if ((rc = voxelize_launch(va, st))) return rc;
+        if ((rc = tm->begin(3, st)) || (rc = voxelize_launch(va, st)) || (rc = tm->end(st))) return rc;
         if ((rc = fill_meta_launch(B, N, grad ? ax : nullptr, s->Nq.p, s->code.p, s->eps_off.p, s->G_off.p, st)))
             return rc;
         Workspace* ws = nullptr;
@@ -222,18 +254,19 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         a.G = s->G.p; a.G_off = s->G_off.p; a.order = nullptr; a.tol = s->fdm_tol; a.inner_floor = 1e-5f;
         a.max_inner = 2000; a.max_rounds = 12; a.iters = nullptr; a.relres = nullptr;
         a.y_ws = ws->y_ws; a.s_ws = ws->s_ws; a.counters = ws->counters;
-        if ((rc = fdm_launch(a, ws->fdm_grid, st))) return rc;
+        if ((rc = tm->begin(4, st)) || (rc = fdm_launch(a, ws->fdm_grid, st)) || (rc = tm->end(st))) return rc;
         if (grad && (rc = center_eps_launch(B, s->Nq.p, ax, s->eps.p, s->eps_off.p, s->cen.p, st))) return rc;
         FlatSampleArgs fa{B, s->Nq.p, grad ? ax : nullptr, s->G.p, s->G_off.p, grad ? s->cen.p : nullptr,
                           s->seed, s->counter, out};
-        if ((rc = sample_flat_launch(fa, st))) return rc;
+        if ((rc = tm->begin(5, st)) || (rc = sample_flat_launch(fa, st)) || (rc = tm->end(st))) return rc;
         s->counter += (uint64_t)B;
         return 0;
     }
     if (s->backend == DRW_BACKEND_NN) {
         QueryArgs q{};
         q.B = B; q.layers = L; q.cuboids = C; q.axis = grad ? ax : nullptr; q.out = out;
-        q.seed = s->seed; q.counter = s->counter;
+        q.face_out = dg.face; q.probs_out = dg.probs; q.weight_out = dg.weight; q.kern_out = dg.kern;
+        q.seed = s->seed; q.counter = s->counter; q.timer = &s->timer;
         int rc = query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, st);
         if (rc) return rc;
         s->counter += (uint64_t)B;
@@ -264,6 +297,44 @@ int drw_solver_sample_device(drw_solver* s, int B, const float* layers, const fl
     if (s->kind == DRW_POISSON && axis) warn_axis_ignored();
     DRW_HIP(hipSetDevice(s->device));
     return solver_run(s, B, layers, cuboids, axis, out, stream ? (hipStream_t)stream : s->stream);
+}
+
+int drw_solver_sample_ex(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                         float* out, int32_t* face_out, float* probs_out, float* weight_out, float* kern_out,
+                         void* stream)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "sample_ex: null solver");
+    if (B < 0) return set_error(DRW_EINVAL, "sample_ex: B < 0");
+    if (B == 0) return 0;
+    if (!layers || !cuboids || !out) return set_error(DRW_EINVAL, "sample_ex: null pointer");
+    if (s->kind == DRW_GRADIENT && !axis) return set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    if (s->backend != DRW_BACKEND_NN) return set_error(DRW_ESTATE, "sample_ex: network backend not loaded");
+    DRW_HIP(hipSetDevice(s->device));
+    Diag dg;
+    dg.face = face_out; dg.probs = probs_out; dg.weight = weight_out; dg.kern = kern_out;
+    return solver_run(s, B, layers, cuboids, s->kind == DRW_GRADIENT ? axis : nullptr, out,
+                      stream ? (hipStream_t)stream : s->stream, dg);
+}
+
+int drw_solver_eval_kernels(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                            float* probs_out, float* weight_out, float* kern_out, void* stream)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "eval_kernels: null solver");
+    if (B < 0) return set_error(DRW_EINVAL, "eval_kernels: B < 0");
+    if (B == 0) return 0;
+    if (!layers || !cuboids || !probs_out || !kern_out) return set_error(DRW_EINVAL, "eval_kernels: null pointer");
+    if (s->kind == DRW_GRADIENT && (!axis || !weight_out))
+        return set_error(DRW_EINVAL, "eval_kernels: gradient solvers need axis and weight_out");
+    if (s->backend != DRW_BACKEND_NN || !s->nets) return set_error(DRW_ESTATE, "eval_kernels: network backend not loaded");
+    DRW_HIP(hipSetDevice(s->device));
+    QueryArgs q{};
+    q.B = B; q.full = 1; q.layers = layers; q.cuboids = cuboids;
+    q.axis = s->kind == DRW_GRADIENT ? axis : nullptr;
+    q.probs_out = probs_out; q.weight_out = weight_out; q.kern_out = kern_out;
+    q.seed = s->seed; q.counter = s->counter;
+    return query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, stream ? (hipStream_t)stream : s->stream);
 }
 
 int drw_solver_sample(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,

@@ -1,0 +1,129 @@
+// Device building blocks of the network queries: on-the-fly voxelization of
+// the structure description, index remaps, MFMA fragments, GELU.
+#pragma once
+
+#include "common.h"
+
+namespace drw {
+
+typedef _Float16 half_t;
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kN = kNetN;            // 23
+constexpr int kNN = kN * kN;         // 529
+constexpr int kNNN = kNN * kN;       // 12167
+constexpr int kPix = 544;            // 529 pixels padded to 34 MFMA row tiles
+
+// ------------------------------------------------------------------------
+// Structure description staged for voxelization.  fill_structure_kernel
+// (inference_cpp/src/cuda_kernels.cu:21-84) picks the last z-layer with
+// min_z <= z_c and lets the first cuboid whose closed box holds the voxel
+// centre override it.  Both tests are separable per axis, so for every grid
+// index we keep the 15-bit set of cuboids whose x (y, z) interval holds that
+// coordinate; the voxel value is cv[ctz(bx & by & bz)] or the layer value.
+// ------------------------------------------------------------------------
+struct VoxDesc {
+    uint16_t bx[kN], by[kN], bz[kN];
+    float lz[kN];          // layer value at each z (normalised)
+    float cv[kMaxCuboids]; // cuboid values (normalised)
+    float maxv;            // per-query max eps (dnn_solver.cpp:274-282)
+};
+
+// Builds the description of query b with one wave (lanes 0..63 of the
+// calling wave); other waves must not call it.  Normalises by the max.
+__device__ inline void vox_desc_build(VoxDesc& d, const float* layers, const float* cuboids, int b)
+{
+    const int lane = threadIdx.x & 63;
+    const float* Ly = layers + (size_t)b * kMaxLayers * 2;
+    const float* Cb = cuboids + (size_t)b * kMaxCuboids * 7;
+    // max over the 15 layer eps and 15 cuboid eps (padding values included)
+    float m = lane < kMaxLayers ? Ly[2 * lane + 1] : (lane < kMaxLayers + kMaxCuboids ? Cb[7 * (lane - kMaxLayers) + 6]
+                                                                                         : -3.0e38f);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    // number of live cuboids (list ends at the first padded minx)
+    int ncub = kMaxCuboids;
+    for (int c = 0; c < kMaxCuboids; c++)
+        if (Cb[7 * c] == kPad) { ncub = c; break; }
+    if (lane < kN) {
+        const float cc = (2.0f * (float)lane + 1.0f) / (float)kN - 1.0f;
+        uint32_t mx = 0, my = 0, mz = 0;
+        for (int c = 0; c < ncub; c++) {
+            const float* q = Cb + 7 * c;
+            if (cc >= q[0] && cc <= q[3]) mx |= 1u << c;
+            if (cc >= q[1] && cc <= q[4]) my |= 1u << c;
+            if (cc >= q[2] && cc <= q[5]) mz |= 1u << c;
+        }
+        d.bx[lane] = (uint16_t)mx;
+        d.by[lane] = (uint16_t)my;
+        d.bz[lane] = (uint16_t)mz;
+        int li = 0;
+        for (int l = 0; l < kMaxLayers; l++) {
+            const float mzv = Ly[2 * l];
+            if (mzv == kPad) break;
+            if (cc >= mzv) li = l; else break;
+        }
+        d.lz[lane] = Ly[2 * li + 1] / m;
+    }
+    if (lane < kMaxCuboids) d.cv[lane] = Cb[7 * lane + 6] / m;
+    if (lane == 0) d.maxv = m;
+}
+
+// Normalised permittivity of voxel (z, y, x) of the query cube.
+__device__ __forceinline__ float vox_value(const VoxDesc& d, int z, int y, int x)
+{
+    const uint32_t hit = (uint32_t)d.bx[x] & (uint32_t)d.by[y] & (uint32_t)d.bz[z];
+    return hit ? d.cv[__builtin_ctz(hit)] : d.lz[z];
+}
+
+// ------------------------------------------------------------------------
+// Index remaps, all as out[z][y][x] = in[src] (cuda_kernels.cu:299-486 and
+// the rot90 of dnn_solver_gradient.cpp:228-238).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ void remap_poisson(int f, int& z, int& y, int& x)
+{
+    const int Z = z, Y = y, X = x;
+    switch (f) {
+    case 1: z = kN - 1 - Z; break;
+    case 2: z = Y; y = Z; break;
+    case 3: z = Y; y = kN - 1 - Z; break;
+    case 4: z = Y; y = X; x = Z; break;
+    case 5: z = Y; y = X; x = kN - 1 - Z; break;
+    default: break;
+    }
+}
+
+__device__ __forceinline__ void remap_gradient(int f, int& z, int& y, int& x)
+{
+    const int Z = z, Y = y, X = x;
+    switch (f) {
+    case 0: z = kN - 1 - Z; break;
+    case 3: y = kN - 1 - Y; break;
+    case 4: y = X; x = Y; break;
+    case 5: y = X; x = kN - 1 - Y; break;
+    default: break;
+    }
+}
+
+__device__ __forceinline__ void remap_axis(int axis, int& z, int& y, int& x)
+{
+    const int Z = z, Y = y, X = x;
+    if (axis == 0) { z = kN - 1 - X; x = Z; }
+    else if (axis == 1) { z = kN - 1 - Y; y = Z; }
+}
+
+// ------------------------------------------------------------------------
+// GELU, exact erf form (torch.nn.GELU default), via the device erff.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+
+// MFMA 16x16x16 fp16 -> fp32.  A: lane l holds row l%16, k = 4(l/16)..+3;
+// B: lane l holds column l%16, k = 4(l/16)..+3; C: lane l holds rows
+// 4(l/16)..+3 of column l%16.
+__device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+}  // namespace drw

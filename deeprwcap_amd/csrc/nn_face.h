@@ -1,0 +1,40 @@
+// Device weight views and launch interface of the face-predictor kernel.
+#pragma once
+
+#include "common.h"
+#include "nn_common.h"
+
+namespace drw {
+
+// Folded (batch-norm absorbed) weights of one FaceSolver / FaceSolverPlus.
+// Layer l of the 1x1 convolutions: l = 0 is z_weight (25 -> C0), l = i + 1
+// the pointwise conv of block i.  MFMA layers use frag[l] (B fragments,
+// [n_tile][k_tile][64 lanes] of half4), the small VALU layers pw_w[l]
+// ([cout][cin] fp32).  dw_w[i] is [cin][9] (ky-major), dw_b[i] [cin].
+struct FaceNetDev {
+    const half4_t* frag[9];
+    const float* pw_w[9];
+    const float* pw_bias[9];
+    const float* dw_w[8];
+    const float* dw_b[8];
+    const float* head_w;
+    float head_b;
+};
+
+struct FaceQueryArgs {
+    int B;                   // queries
+    int full;                // 1: evaluate all 6 faces, write kernels only
+    const float* layers;     // [B, 15, 2]
+    const float* cuboids;    // [B, 15, 7]
+    const int32_t* face;     // [B] selected face (sample mode)
+    const int32_t* axis;     // [B] gradient axis (null for Poisson)
+    const float* weight;     // [B] predicted W (gradient)
+    float* out;              // [B, 3 | 5] (sample mode)
+    float* kern_out;         // [items, 529] normalised (post-processed) kernels, optional in sample mode
+    uint64_t seed, counter;
+};
+
+// mode 0: PoissonPredictor, 1: Gradient1Predictor, 2: Gradient2Predictor.
+int face_query_launch(int mode, const FaceQueryArgs& a, const FaceNetDev& w, hipStream_t stream);
+
+}  // namespace drw

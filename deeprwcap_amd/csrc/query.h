@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.h"
+#include "runtime.h"
 
 namespace drw {
 
@@ -16,11 +17,18 @@ void net_weights_free(NetWeights* w);
 
 struct QueryArgs {
     int B;
+    int full;               // 1: all six faces, kernels only (no draws)
     const float* layers;    // [B, 15, 2]
     const float* cuboids;   // [B, 15, 7]
     const int32_t* axis;    // [B] (gradient) or null (Poisson)
-    float* out;             // [B, 3] or [B, 5]
+    float* out;             // [B, 3] or [B, 5] (sample mode)
+    // optional diagnostics (device pointers, may be null)
+    float* probs_out;       // [B, 6] selector face probabilities
+    int32_t* face_out;      // [B] drawn face
+    float* weight_out;      // [B] predicted W (gradient)
+    float* kern_out;        // sample: [B, 529] kernel of the drawn face; full: [B, 6, 529]
     uint64_t seed, counter;
+    StageTimer* timer;      // optional
 };
 
 // Enqueues the fused network query; *scratch (grown as needed, *cap bytes)

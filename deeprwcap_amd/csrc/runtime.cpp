@@ -56,4 +56,59 @@ int get_workspace(hipStream_t stream, Workspace** out)
     return 0;
 }
 
+static int take_event(std::vector<hipEvent_t>& pool, hipEvent_t* e)
+{
+    if (!pool.empty()) {
+        *e = pool.back();
+        pool.pop_back();
+        return 0;
+    }
+    DRW_HIP(hipEventCreate(e));
+    return 0;
+}
+
+int StageTimer::begin(int stage, hipStream_t st)
+{
+    if (!enabled) return 0;
+    if (stage < 0 || stage >= kStages) return set_error(-2, "StageTimer: bad stage %d", stage);
+    int rc;
+    if ((rc = take_event(pool, &open.a)) || (rc = take_event(pool, &open.b))) return rc;
+    open.stage = stage;
+    is_open = true;
+    DRW_HIP(hipEventRecord(open.a, st));
+    return 0;
+}
+
+int StageTimer::end(hipStream_t st)
+{
+    if (!enabled || !is_open) return 0;
+    DRW_HIP(hipEventRecord(open.b, st));
+    pending.push_back(open);
+    is_open = false;
+    return 0;
+}
+
+int StageTimer::collect()
+{
+    for (auto& p : pending) {
+        DRW_HIP(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        DRW_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        total_ms[p.stage] += ms;
+        count[p.stage] += 1;
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+    return 0;
+}
+
+void StageTimer::release()
+{
+    for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+    pending.clear();
+    pool.clear();
+}
+
 }  // namespace drw

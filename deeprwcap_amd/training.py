@@ -1,0 +1,273 @@
+"""Weights pipeline: FDM dataset on the GPU, network training, export.
+
+The reference ships its trained networks only as compiled TensorRT engines,
+so the weights are produced here, following the reference pipeline
+(ggft/src/main.cpp dataset driver -> training_pytorch/src/dataset.py targets
+-> training_pytorch/src/train.py losses):
+
+* cubes: seeded structure descriptions (deeprwcap_amd.structures) voxelized
+  at N = 23 by the library; kernels G (Poisson, GFT) and G^z (gradient, WVTZ)
+  from the library's batched FDM solve;
+* targets (dataset.py:108-120): face distribution = per-face sums of G and
+  per-face normalised kernels; signed gradient normalised by the face's
+  sum |G^z|; face weights [face_tot / sum, sum face_tot];
+* augmentation by the query's own face rotations, whose targets follow from
+  FDM equivariance: G(rotate_poisson_f(cube))[face 0] = G(cube)[face f], and
+  G^z(rotate_gradient_f(cube))[face 1 or 2] = +-G^z(cube)[face f];
+* losses: KL for distributions (PoissonSelector, PoissonPredictor), MSE for
+  the signed gradients, KL + MSE on W for GradientSelectorWeight
+  (main.py:43-84 "mainloss"); AdamW with cosine schedule;
+* the network input is eps / max(eps) per cube (train.py:59-60).
+
+Module parameter names equal the reference state-dict keys, so the exported
+.drwnet files carry the same tensor names the reference checkpoints have.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+import torch.nn as nn
+import torch.nn.functional as F
+
+from . import device as D
+from . import structures as S
+from . import weights as W
+
+NET_N = 23
+
+
+# ------------------------------------------------------------------ modules
+class _DSB(nn.Module):
+    def __init__(self, ci, co, d):
+        super().__init__()
+        self.block = nn.Sequential(nn.Conv2d(ci, ci, 3, padding=d, dilation=d, groups=ci, bias=False),
+                                   nn.BatchNorm2d(ci), nn.GELU(), nn.Conv2d(ci, co, 1, bias=False),
+                                   nn.BatchNorm2d(co), nn.GELU())
+
+    def forward(self, x):
+        return self.block(x)
+
+
+class _Solver(nn.Module):
+    def __init__(self, c0, blocks):
+        super().__init__()
+        self.z_weight = nn.Sequential(nn.Conv2d(25, c0, 1, bias=False), nn.BatchNorm2d(c0), nn.GELU())
+        self.blocks = nn.Sequential(*[_DSB(ci, co, d) for ci, co, d in blocks])
+        self.head = nn.Conv2d(blocks[-1][1], 1, 1, bias=True)
+        y = torch.linspace(0, 1, NET_N)
+        yy, xx = torch.meshgrid(y, y, indexing="ij")
+        self.register_buffer("pe", torch.stack((xx, yy), 0).unsqueeze(0), persistent=False)
+
+    def forward(self, x):
+        x = torch.cat((x, self.pe.expand(x.shape[0], 2, NET_N, NET_N)), 1)
+        return self.head(self.blocks(self.z_weight(x)))
+
+
+class FaceNet(nn.Module):
+    """FacePredictor / SignedFacePredictor (raw head output; the normalising
+    output head is applied by the caller)."""
+
+    def __init__(self, plus=False):
+        super().__init__()
+        self.plus = plus
+        self.solver = _Solver(*(W.FACE_SOLVER_PLUS if plus else W.FACE_SOLVER))
+
+    def forward(self, cube):  # cube [B, 23(z), 23(y), 23(x)]
+        x = cube.permute(0, 2, 1, 3) if self.plus else cube
+        return self.solver(x).squeeze(1)
+
+
+class SelectorNet(nn.Module):
+    def __init__(self, weighted=False):
+        super().__init__()
+        ch = [1, 8, 16, 64] if weighted else [1, 4, 8, 16]
+        layers = []
+        for i in range(3):
+            layers += [nn.Conv3d(ch[i], ch[i + 1], 3, stride=2), nn.BatchNorm3d(ch[i + 1]), nn.GELU()]
+        self.backbone = nn.Sequential(*layers)
+        dims = [512, 256, 128, 7] if weighted else [128, 64, 6]
+        head = [nn.Flatten()]
+        for j in range(len(dims) - 1):
+            head.append(nn.Linear(dims[j], dims[j + 1]))
+            if j < len(dims) - 2:
+                head.append(nn.GELU())
+        self.head = nn.Sequential(*head)
+
+    def forward(self, cube):
+        return self.head(self.backbone(cube[:, None]))  # logits
+
+
+def export_state(model: nn.Module) -> dict:
+    return {k: v.detach().float().cpu().numpy() for k, v in model.state_dict().items()
+            if not k.endswith("num_batches_tracked")}
+
+
+# ------------------------------------------------------------------ dataset
+def fdm_dataset(n: int, seed: int, tol: float = 1e-10, max_boxes: int = 2, chunk: int = 4096, dev="cuda"):
+    """Normalised cubes [n,23,23,23] (float16 on the host) and the FDM kernels
+    G, G^z [n, 6, 529] (float32), computed with the library on the GPU."""
+    cubes = np.empty((n, NET_N, NET_N, NET_N), np.float16)
+    G = np.empty((n, 6, 529), np.float32)
+    Gz = np.empty((n, 6, 529), np.float32)
+    for c0 in range(0, n, chunk):
+        b = min(chunk, n - c0)
+        L, C, _ = S.layered_boxes(b, NET_N, seed=seed * 100003 + c0, max_boxes=max_boxes)
+        pk = D.PackedCubes([NET_N] * b, device=dev)
+        D.voxelize(L, C, fixed_N=NET_N, normalise=True, out=pk.eps.view(b, NET_N, NET_N, NET_N))
+        cubes[c0:c0 + b] = pk.eps.view(b, NET_N, NET_N, NET_N).cpu().numpy().astype(np.float16)
+        for code, dst in ((0, G), (1, Gz)):
+            D.fdm_solve(pk, code, tol=tol, want_stats=False)
+            dst[c0:c0 + b] = pk.G.view(b, 6, 529).float().cpu().numpy()
+    return cubes, G, Gz
+
+
+def _index_rot(cubes, f, kind):
+    """out[z][y][x] = in[src(z,y,x)] by explicit index tables (exact)."""
+    n = NET_N
+    z, y, x = torch.meshgrid(torch.arange(n), torch.arange(n), torch.arange(n), indexing="ij")
+    if kind == "poisson":
+        src = {0: (z, y, x), 1: (n - 1 - z, y, x), 2: (y, z, x), 3: (y, n - 1 - z, x), 4: (y, x, z),
+               5: (y, x, n - 1 - z)}[f]
+    else:
+        src = {0: (n - 1 - z, y, x), 1: (z, y, x), 2: (z, y, x), 3: (z, n - 1 - y, x), 4: (z, x, y),
+               5: (z, x, n - 1 - y)}[f]
+    idx = (src[0] * n * n + src[1] * n + src[2]).reshape(-1).to(cubes.device)
+    return cubes.reshape(cubes.shape[0], -1)[:, idx].reshape(cubes.shape)
+
+
+# ----------------------------------------------------------------- training
+def _kl(pred_logp, target):
+    t = target.clamp_min(1e-12)
+    return (target * (t.log() - pred_logp)).sum(-1).mean()
+
+
+def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log=print, seed=0):
+    torch.manual_seed(seed)
+    n = cubes.shape[0]
+    ntr = int(n * 0.95)
+    Xall = torch.from_numpy(cubes).to(dev)
+    Gp = torch.from_numpy(G).to(dev)
+    Gg = torch.from_numpy(Gz).to(dev)
+    if name in ("PoissonSelector", "GradientSelectorWeight"):
+        model = SelectorNet(weighted=name == "GradientSelectorWeight").to(dev)
+    else:
+        model = FaceNet(plus=name == "Gradient2Predictor").to(dev)
+    faces = {"PoissonPredictor": [0, 1, 2, 3, 4, 5], "Gradient1Predictor": [0, 1],
+             "Gradient2Predictor": [2, 3, 4, 5]}.get(name, [None])
+    opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-10)
+    steps_per_epoch = max(1, (ntr * len(faces)) // batch)
+    total = epochs * steps_per_epoch
+    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=total, pct_start=0.1)
+
+    def make_batch(idx, f):
+        x = Xall[idx].float()
+        if name == "PoissonSelector":
+            tgt = Gp[idx].sum(-1)
+            return x, tgt
+        if name == "GradientSelectorWeight":
+            tot = Gg[idx].abs().sum(-1)
+            s = tot.sum(-1, keepdim=True)
+            return x, torch.cat([tot / s, s], 1)
+        if name == "PoissonPredictor":
+            k = Gp[idx, f]
+            return _index_rot(x, f, "poisson"), k / k.sum(-1, keepdim=True)
+        # gradient predictors: target face of the rotated cube
+        k = Gg[idx, f]
+        k = -k if f == 0 else k  # flip z of an odd-N cube negates the z-gradient kernel
+        return _index_rot(x, f, "gradient"), k / k.abs().sum(-1, keepdim=True)
+
+    def loss_of(x, tgt):
+        out = model(x)
+        if name == "PoissonSelector":
+            return _kl(F.log_softmax(out, -1), tgt)
+        if name == "GradientSelectorWeight":
+            return _kl(F.log_softmax(out[:, :6], -1), tgt[:, :6]) + 0.1 * F.mse_loss(out[:, 6], tgt[:, 6])
+        y = out.reshape(out.shape[0], -1)
+        if name == "PoissonPredictor":
+            p = F.relu(y) + 1e-10
+            p = p / p.sum(-1, keepdim=True)
+            return _kl(p.log(), tgt)
+        p = y / y.abs().sum(-1, keepdim=True)
+        return F.mse_loss(p, tgt) * 529.0 * 50.0
+
+    t0 = time.time()
+    for ep in range(epochs):
+        model.train()
+        perm = torch.randperm(ntr, device=dev)
+        tot_l, nb = 0.0, 0
+        for st in range(steps_per_epoch):
+            f = faces[st % len(faces)]
+            idx = perm[(st * batch) % ntr:(st * batch) % ntr + batch]
+            if idx.numel() < 2:
+                continue
+            x, tgt = make_batch(idx, f)
+            loss = loss_of(x, tgt)
+            opt.zero_grad(set_to_none=True)
+            loss.backward()
+            torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
+            opt.step()
+            sched.step()
+            tot_l += float(loss)
+            nb += 1
+        if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
+            log(f"{name} epoch {ep + 1}/{epochs} loss {tot_l / max(nb, 1):.5f} ({time.time() - t0:.0f}s)")
+    model.eval()
+    return model
+
+
+def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda"):
+    """Mean per-face normalised-kernel L1 on held-out cubes."""
+    faces = {"PoissonPredictor": range(6), "Gradient1Predictor": [0, 1], "Gradient2Predictor": [2, 3, 4, 5]}[name]
+    Xall = torch.from_numpy(cubes[idx]).to(dev).float()
+    errs = []
+    with torch.no_grad():
+        for f in faces:
+            if name == "PoissonPredictor":
+                k = torch.from_numpy(G[idx, f]).to(dev)
+                tgt = k / k.sum(-1, keepdim=True)
+                y = model(_index_rot(Xall, f, "poisson")).reshape(len(idx), -1)
+                p = F.relu(y) + 1e-10
+                p = p / p.sum(-1, keepdim=True)
+            else:
+                k = torch.from_numpy(Gz[idx, f]).to(dev)
+                k = -k if f == 0 else k
+                tgt = k / k.abs().sum(-1, keepdim=True)
+                y = model(_index_rot(Xall, f, "gradient")).reshape(len(idx), -1)
+                p = y / y.abs().sum(-1, keepdim=True)
+            errs.append((p - tgt).abs().sum(-1).mean().item())
+    return float(np.mean(errs))
+
+
+def train_all(out_dir, n_cubes=40000, epochs=30, seed=0, log=print, dev="cuda"):
+    t0 = time.time()
+    cubes, G, Gz = fdm_dataset(n_cubes, seed, dev=dev)
+    log(f"dataset: {n_cubes} cubes, FDM kernels in {time.time() - t0:.1f}s")
+    states, report = {}, {}
+    ho = np.arange(int(n_cubes * 0.95), n_cubes)
+    for name in W.MODEL_ORDER:
+        m = train_net(name, cubes, G, Gz, epochs=epochs, dev=dev, log=log)
+        states[name] = export_state(m)
+        if name.endswith("Predictor"):
+            report[name] = evaluate_face_l1(name, m, cubes, G, Gz, ho, dev=dev)
+            log(f"{name}: held-out per-face L1 {report[name]:.4f}")
+    path = W.export_model_dir(out_dir, states)
+    log(f"exported {path} in {time.time() - t0:.0f}s")
+    return path, report
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="FDM dataset + training + .drwnet export")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--n", type=int, default=40000)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+
+    def _log(msg):
+        print(msg, flush=True)
+
+    train_all(args.out, n_cubes=args.n, epochs=args.epochs, seed=args.seed, log=_log)

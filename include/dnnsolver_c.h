@@ -102,6 +102,15 @@ int drw_solver_get_counter(const drw_solver* s, uint64_t* counter);
 /* Output floats per query: 3 (Poisson) or 5 (gradient). */
 int drw_solver_output_width(const drw_solver* s);
 
+/* Per-stage kernel timing with HIP events on the launch stream (the
+ * counterpart of the walker's "GPU Sampler Statistics",
+ * executable/run_script.py:108-162).  Stages: 0 selector backbone,
+ * 1 selector head, 2 face predictor, 3 voxelize, 4 FDM solve, 5 flat
+ * sampler.  stage_times waits for the recorded events and returns the
+ * accumulated milliseconds and launch counts of 8 stages (reset != 0 clears). */
+int drw_solver_set_timing(drw_solver* s, int enable);
+int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int reset);
+
 /* sample(layers, cuboids[, axis]) with host pointers; blocks until `out`
  * holds B * width floats.  B = size(layers) / 30 in the reference
  * (dnn_solver_poisson.cpp:40).  Gradient solvers require axis (the reference
@@ -112,6 +121,25 @@ int drw_solver_sample(drw_solver* s, int B, const float* layers, const float* cu
 /* Same with device pointers, enqueued on `stream` (NULL: the solver's own). */
 int drw_solver_sample_device(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
                              float* out, void* stream);
+
+/* Network backend with diagnostics (device pointers, each optional):
+ * face_out [B] drawn face, probs_out [B][6] selector probabilities,
+ * weight_out [B] predicted W (gradient), kern_out [B][529] normalised kernel
+ * of the drawn face (post-processed for gradients) from which the tile was
+ * drawn.  Replaces the intermediate tensors of samplePoissonFunction /
+ * sampleGradient (dnn_solver_poisson.cpp:86-111, dnn_solver_gradient.cpp:
+ * 111-164). */
+int drw_solver_sample_ex(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                         float* out, int32_t* face_out, float* probs_out, float* weight_out, float* kern_out,
+                         void* stream);
+
+/* Full transition kernels from the networks (no draws), for accuracy checks
+ * against the FDM judge: probs_out [B][6], kern_out [B][6][529] (face f of
+ * the query's frame, tile t = 23 b + a in GGFT panel order), weight_out [B]
+ * (gradient; required there).  Poisson: G[f][t] = probs[f] * kern[f][t];
+ * gradient: G[remap(f)][t] = W * probs[f] * kern[f][t]. */
+int drw_solver_eval_kernels(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                            float* probs_out, float* weight_out, float* kern_out, void* stream);
 
 #ifdef __cplusplus
 }

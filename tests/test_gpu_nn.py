@@ -1,0 +1,110 @@
+"""GPU parity of the network queries (fp16 MFMA, fp32 accumulate) against the
+fp32 PyTorch restatement on the same weights (the reference modules' seeded
+weights recorded in tests/golden/nets_golden.npz), and bit-exact hop draws
+against the CPU sampler given the GPU's own probabilities and uniforms."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+L1_TOL = 1e-3  # per-kernel L1 of normalised kernels vs the fp32 forward
+
+
+@pytest.fixture(scope="module")
+def models_txt(tmp_path_factory):
+    from deeprwcap_amd import weights as W
+    d = np.load(os.path.join(ROOT, "tests", "golden", "nets_golden.npz"))
+    states = {n: W.state_from_npz(d, n) for n in W.MODEL_ORDER}
+    return W.export_model_dir(str(tmp_path_factory.mktemp("models")), states)
+
+
+@pytest.fixture(scope="module")
+def params():
+    import torch
+    from oracle import nets_ref as R
+    d = np.load(os.path.join(ROOT, "tests", "golden", "nets_golden.npz"))
+    return {n: {k: torch.from_numpy(v) for k, v in R.params_from_npz(d, n).items()} for n in R.ARCH}
+
+
+def _solver(kind, models_txt):
+    from deeprwcap_amd.dnnsolver import DNNSolverGrad, DNNSolverPoisson
+    s = (DNNSolverPoisson if kind == 0 else DNNSolverGrad)(0)
+    s.loadModels(models_txt=models_txt)
+    return s
+
+
+def _eval(s, L, C, axis):
+    import torch
+    from deeprwcap_amd import _lib
+    B = L.shape[0]
+    dL = torch.from_numpy(L).cuda()
+    dC = torch.from_numpy(C).cuda()
+    dA = None if axis is None else torch.from_numpy(axis.astype(np.int32)).cuda()
+    probs = torch.empty((B, 6), device="cuda")
+    wts = torch.empty(B, device="cuda")
+    kern = torch.empty((B, 6, 529), device="cuda")
+    _lib.check(_lib.lib().drw_solver_eval_kernels(s.handle, B, dL.data_ptr(), dC.data_ptr(),
+                                                   None if dA is None else dA.data_ptr(), probs.data_ptr(),
+                                                   wts.data_ptr(), kern.data_ptr(), None), "eval_kernels")
+    torch.cuda.synchronize()
+    return probs.cpu().numpy(), wts.cpu().numpy(), kern.cpu().numpy()
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_nn_kernels_match_fp32(kind, models_txt, params):
+    from deeprwcap_amd import structures as S
+    from oracle import nets_ref as R
+    B = 24
+    L, C, _ = S.layered_boxes(B, 23, seed=40 + kind)
+    axis = (np.arange(B) % 3).astype(np.int32) if kind == 1 else None
+    s = _solver(kind, models_txt)
+    probs, wts, kern = _eval(s, L, C, axis)
+    rp, rw, rk = R.query_kernels(kind, L, C, axis, params)
+    l1 = np.abs(kern - rk).sum(-1)
+    print(f"kind {kind}: probs max err {np.abs(probs - rp).max():.2e}, kernel L1 max {l1.max():.2e} "
+          f"mean {l1.mean():.2e}", flush=True)
+    assert np.abs(probs - rp).max() < 2e-3
+    assert l1.max() < L1_TOL
+    if kind == 1:
+        assert np.abs(wts - rw).max() < 2e-3 * max(1.0, np.abs(rw).max())
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_nn_hops_bitexact(kind, models_txt, orc):
+    import torch
+    from deeprwcap_amd import _lib, structures as S
+    B = 64
+    L, C, _ = S.layered_boxes(B, 23, seed=60 + kind)
+    axis = (np.arange(B) % 3).astype(np.int32)
+    s = _solver(kind, models_txt)
+    seed, counter = 987654321, 4242
+    s.set_seed(seed, counter)
+    width = 3 if kind == 0 else 5
+    dL, dC = torch.from_numpy(L).cuda(), torch.from_numpy(C).cuda()
+    dA = torch.from_numpy(axis).cuda()
+    out = torch.empty((B, width), device="cuda")
+    face = torch.empty(B, dtype=torch.int32, device="cuda")
+    probs = torch.empty((B, 6), device="cuda")
+    wts = torch.empty(B, device="cuda")
+    kern = torch.empty((B, 529), device="cuda")
+    _lib.check(_lib.lib().drw_solver_sample_ex(s.handle, B, dL.data_ptr(), dC.data_ptr(),
+                                                dA.data_ptr() if kind == 1 else None, out.data_ptr(),
+                                                face.data_ptr(), probs.data_ptr(), wts.data_ptr(), kern.data_ptr(),
+                                                None), "sample_ex")
+    torch.cuda.synchronize()
+    out, face, probs, wts, kern = (t.cpu().numpy() for t in (out, face, probs, wts, kern))
+    assert s.counter == counter + B
+    cubes, mx = orc.voxelize(L, C, 23, normalise=True)
+    for b in range(B):
+        u = orc.query_uniforms(seed, counter + b)
+        f = orc.cdf_draw(probs[b], int(u[0]))
+        assert f == face[b]
+        t = orc.cdf_draw(kern[b], int(u[1]))
+        fo = f if kind == 0 else {0: {0: 4, 1: 5, 4: 1, 5: 0}, 1: {0: 2, 1: 3, 2: 1, 3: 0}, 2: {}}[int(axis[b])].get(f, f)
+        np.testing.assert_array_equal(out[b, :3], orc.locate(23, t, fo))
+        if kind == 1:
+            assert out[b, 3] == wts[b] * np.sign(kern[b, t])
+            assert out[b, 4] == np.float32(cubes[b, 11, 11, 11] * mx[b])
