@@ -210,7 +210,7 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
             torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
             opt.step()
             sched.step()
-            tot_l += float(loss)
+            tot_l += loss.item()
             nb += 1
         if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
             log(f"{name} epoch {ep + 1}/{epochs} loss {tot_l / max(nb, 1):.5f} ({time.time() - t0:.0f}s)")
