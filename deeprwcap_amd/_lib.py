@@ -35,8 +35,14 @@ SIGNATURES = {
     "drw_solver_stage_times": (_c_int, [_c_vp, _c_vp, _c_vp, _c_int]),
     "drw_solver_sample": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_device": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "drw_solver_sample_ids": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_ex": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
                                       _c_vp]),
+    "drw_server_create": (_c_int, [_c_int, ctypes.c_char_p, _c_vp, ctypes.POINTER(_c_vp)]),
+    "drw_server_submit": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_u64, ctypes.POINTER(_c_u64)]),
+    "drw_server_wait": (_c_int, [_c_vp, _c_u64, _c_vp, _c_int]),
+    "drw_server_stats_get": (_c_int, [_c_vp, _c_vp, _c_int]),
+    "drw_server_destroy": (None, [_c_vp]),
     "drw_solver_eval_kernels": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
 }
 

@@ -115,7 +115,7 @@ int drw_sample_flat(int B, const int32_t* N, const int32_t* axis, const double* 
     clear_error();
     if (B < 0) return set_error(DRW_EINVAL, "drw_sample_flat: B < 0");
     if (!N || !G || !G_off || !out) return set_error(DRW_EINVAL, "drw_sample_flat: null pointer");
-    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out};
+    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out, nullptr};
     return sample_flat_launch(a, (hipStream_t)stream);
 }
 
@@ -225,6 +225,7 @@ int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int
 }
 
 struct Diag {
+    const uint64_t* qid = nullptr;  // per-query Philox counters (device)
     int32_t* face = nullptr;
     float* probs = nullptr;
     float* weight = nullptr;
@@ -257,7 +258,7 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         if ((rc = tm->begin(4, st)) || (rc = fdm_launch(a, ws->fdm_grid, st)) || (rc = tm->end(st))) return rc;
         if (grad && (rc = center_eps_launch(B, s->Nq.p, ax, s->eps.p, s->eps_off.p, s->cen.p, st))) return rc;
         FlatSampleArgs fa{B, s->Nq.p, grad ? ax : nullptr, s->G.p, s->G_off.p, grad ? s->cen.p : nullptr,
-                          s->seed, s->counter, out};
+                          s->seed, s->counter, out, dg.qid};
         if ((rc = tm->begin(5, st)) || (rc = sample_flat_launch(fa, st)) || (rc = tm->end(st))) return rc;
         s->counter += (uint64_t)B;
         return 0;
@@ -266,6 +267,7 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         QueryArgs q{};
         q.B = B; q.layers = L; q.cuboids = C; q.axis = grad ? ax : nullptr; q.out = out;
         q.face_out = dg.face; q.probs_out = dg.probs; q.weight_out = dg.weight; q.kern_out = dg.kern;
+        q.qid = dg.qid;
         q.seed = s->seed; q.counter = s->counter; q.timer = &s->timer;
         int rc = query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, st);
         if (rc) return rc;
@@ -315,6 +317,25 @@ int drw_solver_sample_ex(drw_solver* s, int B, const float* layers, const float*
     dg.face = face_out; dg.probs = probs_out; dg.weight = weight_out; dg.kern = kern_out;
     return solver_run(s, B, layers, cuboids, s->kind == DRW_GRADIENT ? axis : nullptr, out,
                       stream ? (hipStream_t)stream : s->stream, dg);
+}
+
+int drw_solver_sample_ids(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                          const uint64_t* qid, float* out, void* stream)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "sample_ids: null solver");
+    if (B < 0) return set_error(DRW_EINVAL, "sample_ids: B < 0");
+    if (B == 0) return 0;
+    if (!layers || !cuboids || !out || !qid) return set_error(DRW_EINVAL, "sample_ids: null pointer");
+    if (s->kind == DRW_GRADIENT && !axis) return set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    DRW_HIP(hipSetDevice(s->device));
+    Diag dg;
+    dg.qid = qid;
+    const uint64_t keep = s->counter;
+    const int rc = solver_run(s, B, layers, cuboids, s->kind == DRW_GRADIENT ? axis : nullptr, out,
+                              stream ? (hipStream_t)stream : s->stream, dg);
+    s->counter = keep;  // per-query counters given: the solver's own sequence is untouched
+    return rc;
 }
 
 int drw_solver_eval_kernels(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,

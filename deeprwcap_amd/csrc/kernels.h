@@ -27,6 +27,7 @@ struct FlatSampleArgs {
     const float* center_eps;   // [B] (gradient only; optional)
     uint64_t seed, counter;    // Philox key and first query counter
     float* out;                // [B, 3] or [B, 5]
+    const uint64_t* qid;       // [B] per-query counters (optional; else counter + b)
 };
 int sample_flat_launch(const FlatSampleArgs& a, hipStream_t stream);
 

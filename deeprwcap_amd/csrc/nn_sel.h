@@ -29,6 +29,7 @@ struct SelHeadArgs {
     float* weight;         // [B] raw W logit (FaceSelectorWeight, optional)
     int32_t* face;         // [B] drawn face (optional)
     uint64_t seed, counter;
+    const uint64_t* qid;   // [B] per-query counters (optional; else counter + b)
 };
 
 int selector_backbone_launch(bool weighted, const SelBackboneArgs& ba, const SelNetDev& w, hipStream_t stream);

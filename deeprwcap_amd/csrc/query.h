@@ -28,6 +28,7 @@ struct QueryArgs {
     float* weight_out;      // [B] predicted W (gradient)
     float* kern_out;        // sample: [B, 529] kernel of the drawn face; full: [B, 6, 529]
     uint64_t seed, counter;
+    const uint64_t* qid;    // [B] per-query Philox counters (optional; else counter + b)
     StageTimer* timer;      // optional
 };
 

@@ -1,0 +1,439 @@
+// Serving engine: continuous batching of transition queries of mixed kind and
+// resolution in front of the GPU, built on the public C ABI.
+//
+// The reference serves walker threads through ParallelSampler
+// (inference_cpp/src/parallel_sampler.cpp:42-211): one worker takes up to
+// 2048 tasks, runs one synchronous batch, and only then looks at the queue
+// again.  Here a dispatcher thread keeps up to `max_inflight` batches on the
+// device at once, each on its own HIP stream with pinned staging buffers:
+// requests that arrive while batches are running join the next batch as soon
+// as a slot and the HBM budget allow, so host staging, H2D, kernels and D2H
+// of different batches overlap.  Each request carries its own Philox counter,
+// so its hop is identical whether it runs alone or in a mix.
+//
+// Classes: (kind, route) with route = network at N = 23 when models are
+// loaded, else the FDM fallback at the request's N.  The oldest pending
+// request decides which class is launched next.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/dnnsolver_c.h"
+#include "common.h"
+#include "runtime.h"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr int kIn = drw::kMaxLayers * 2 + drw::kMaxCuboids * 7;  // 135 floats
+constexpr int kRoutes = 4;                                        // NN23, FDM16, FDM21, FDM23
+
+struct Req {
+    uint64_t ticket = 0, qid = 0;
+    int kind = 0, N = 23, axis = 2, cls = 0;
+    float in[kIn];
+    float out[5];
+    int rc = 0;
+    bool done = false;
+    Clock::time_point t_submit, t_done;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    float *h_L = nullptr, *h_C = nullptr, *h_out = nullptr;
+    int32_t* h_ax = nullptr;
+    uint64_t* h_q = nullptr;
+    float *d_L = nullptr, *d_C = nullptr, *d_out = nullptr;
+    int32_t* d_ax = nullptr;
+    uint64_t* d_q = nullptr;
+    drw_solver* solvers[2 * kRoutes] = {};
+    std::vector<Req*> batch;
+    size_t bytes = 0;
+    int cls = -1, launch_rc = 0;
+    bool busy = false;
+};
+
+int route_of(int N, bool nn)
+{
+    if (N == 23 && nn) return 0;
+    return N == 16 ? 1 : N == 21 ? 2 : 3;
+}
+
+int route_N(int r) { return r == 1 ? 16 : r == 2 ? 21 : 23; }
+
+// Device bytes one query of a route occupies while its batch is in flight.
+size_t route_bytes(int r)
+{
+    const size_t io = sizeof(float) * (kIn + 5) + 16;
+    if (r == 0) return io + sizeof(float) * (512 + 8);  // selector features + probabilities
+    const size_t n = (size_t)route_N(r);
+    return io + n * n * n * sizeof(float) + 6 * n * n * sizeof(double) + 64;
+}
+
+}  // namespace
+
+struct drw_server {
+    int device = 0;
+    drw_server_config cfg{};
+    std::string models;
+    bool nn = false;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req*> queue[2 * kRoutes];
+    std::map<uint64_t, std::unique_ptr<Req>> reqs;
+    uint64_t next_ticket = 1;
+    size_t bytes_inflight = 0;
+    bool stop = false;
+    std::thread worker;
+    // statistics
+    std::vector<float> lat_ms;
+    uint64_t completed = 0, batches = 0, batched = 0, inflight_peak = 0;
+    Clock::time_point t0 = Clock::now();
+    std::string fatal;
+
+    int slot_solver(Slot& s, int cls, drw_solver** out);
+    int launch(Slot& s, int cls, std::vector<Req*>& batch);
+    void reap(Slot& s);
+    void loop();
+};
+
+int drw_server::slot_solver(Slot& s, int cls, drw_solver** out)
+{
+    if (!s.solvers[cls]) {
+        const int kind = cls / kRoutes, r = cls % kRoutes;
+        drw_solver* h = nullptr;
+        int rc = drw_solver_create(kind, device, &h);
+        if (rc) return rc;
+        rc = r == 0 ? drw_solver_load_models(h, models.c_str()) : drw_solver_use_fdm(h, route_N(r), cfg.fdm_tol);
+        if (!rc) rc = drw_solver_set_seed(h, cfg.seed, 0);
+        if (rc) {
+            drw_solver_destroy(h);
+            return rc;
+        }
+        s.solvers[cls] = h;
+    }
+    *out = s.solvers[cls];
+    return 0;
+}
+
+int drw_server::launch(Slot& s, int cls, std::vector<Req*>& batch)
+{
+    const int B = (int)batch.size();
+    const int kind = cls / kRoutes;
+    for (int i = 0; i < B; i++) {
+        const Req* q = batch[i];
+        std::memcpy(s.h_L + (size_t)i * 30, q->in, sizeof(float) * 30);
+        std::memcpy(s.h_C + (size_t)i * 105, q->in + 30, sizeof(float) * 105);
+        s.h_ax[i] = q->axis;
+        s.h_q[i] = q->qid;
+    }
+    drw_solver* sol = nullptr;
+    int rc = slot_solver(s, cls, &sol);
+    if (rc) return rc;
+    DRW_HIP(hipMemcpyAsync(s.d_L, s.h_L, sizeof(float) * B * 30, hipMemcpyHostToDevice, s.stream));
+    DRW_HIP(hipMemcpyAsync(s.d_C, s.h_C, sizeof(float) * B * 105, hipMemcpyHostToDevice, s.stream));
+    DRW_HIP(hipMemcpyAsync(s.d_ax, s.h_ax, sizeof(int32_t) * B, hipMemcpyHostToDevice, s.stream));
+    DRW_HIP(hipMemcpyAsync(s.d_q, s.h_q, sizeof(uint64_t) * B, hipMemcpyHostToDevice, s.stream));
+    rc = drw_solver_sample_ids(sol, B, s.d_L, s.d_C, kind == DRW_GRADIENT ? s.d_ax : nullptr, s.d_q, s.d_out, s.stream);
+    if (rc) return rc;
+    const int w = kind == DRW_GRADIENT ? 5 : 3;
+    DRW_HIP(hipMemcpyAsync(s.h_out, s.d_out, sizeof(float) * B * w, hipMemcpyDeviceToHost, s.stream));
+    DRW_HIP(hipEventRecord(s.done, s.stream));
+    return 0;
+}
+
+void drw_server::reap(Slot& s)
+{
+    const int w = s.cls / kRoutes == DRW_GRADIENT ? 5 : 3;
+    const auto now = Clock::now();
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t i = 0; i < s.batch.size(); i++) {
+        Req* q = s.batch[i];
+        if (s.launch_rc == 0) std::memcpy(q->out, s.h_out + i * w, sizeof(float) * w);
+        q->rc = s.launch_rc;
+        q->done = true;
+        q->t_done = now;
+        const float ms = std::chrono::duration<float, std::milli>(now - q->t_submit).count();
+        if (lat_ms.size() < (1u << 22)) lat_ms.push_back(ms);
+    }
+    completed += s.batch.size();
+    batches += 1;
+    batched += s.batch.size();
+    bytes_inflight -= s.bytes;
+    s.batch.clear();
+    s.busy = false;
+    cv_done.notify_all();
+}
+
+void drw_server::loop()
+{
+    (void)hipSetDevice(device);
+    for (;;) {
+        // 1) completions
+        bool progressed = false;
+        int busy = 0;
+        for (auto& s : slots) {
+            if (!s.busy) continue;
+            const hipError_t e = s.launch_rc ? hipSuccess : hipEventQuery(s.done);
+            if (e == hipSuccess) {
+                reap(s);
+                progressed = true;
+            } else if (e != hipErrorNotReady) {
+                s.launch_rc = DRW_EHIP;
+                reap(s);
+                progressed = true;
+            } else {
+                busy++;
+            }
+        }
+        // 2) admissions: the class of the oldest pending request, limited by
+        //    free slots, the batch cap and the HBM budget
+        for (auto& s : slots) {
+            if (s.busy) continue;
+            std::vector<Req*> batch;
+            int cls = -1;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                Clock::time_point oldest = Clock::time_point::max();
+                for (int c = 0; c < 2 * kRoutes; c++)
+                    if (!queue[c].empty() && queue[c].front()->t_submit < oldest) {
+                        oldest = queue[c].front()->t_submit;
+                        cls = c;
+                    }
+                if (cls < 0) break;
+                const size_t per = route_bytes(cls % kRoutes);
+                const size_t budget = (size_t)(cfg.hbm_budget_mb * 1048576.0);
+                size_t room = budget > bytes_inflight ? (budget - bytes_inflight) / per : 0;
+                if (bytes_inflight == 0 && room == 0) room = 1;  // always make progress
+                const size_t n = std::min<size_t>({queue[cls].size(), (size_t)cfg.max_batch, room});
+                if (n == 0) break;  // wait for a batch to finish and free memory
+                for (size_t i = 0; i < n; i++) {
+                    batch.push_back(queue[cls].front());
+                    queue[cls].pop_front();
+                }
+                s.bytes = n * per;
+                bytes_inflight += s.bytes;
+            }
+            s.batch = batch;
+            s.cls = cls;
+            s.busy = true;
+            s.launch_rc = launch(s, cls, s.batch);
+            busy++;
+            progressed = true;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            inflight_peak = std::max<uint64_t>(inflight_peak, (uint64_t)busy);
+        }
+        if (progressed) continue;
+        std::unique_lock<std::mutex> lk(mu);
+        bool pending = false;
+        for (auto& qd : queue) pending |= !qd.empty();
+        if (stop && !pending && busy == 0) return;
+        if (busy > 0) {
+            // batches in flight: poll their events again shortly
+            cv_work.wait_for(lk, std::chrono::microseconds(20));
+        } else {
+            cv_work.wait(lk, [&] {
+                if (stop) return true;
+                for (auto& qd : queue)
+                    if (!qd.empty()) return true;
+                return false;
+            });
+        }
+    }
+}
+
+extern "C" {
+
+int drw_server_create(int device, const char* models_txt, const drw_server_config* cfg, drw_server** out)
+{
+    drw::clear_error();
+    if (!out) return drw::set_error(DRW_EINVAL, "server_create: out is null");
+    *out = nullptr;
+    drw_server_config c{};
+    c.max_batch = 2048;
+    c.max_inflight = 4;
+    c.hbm_budget_mb = 1024.0;
+    c.fdm_tol = 1e-6;
+    c.seed = 0x5eed5eedULL;
+    if (cfg) c = *cfg;
+    if (c.max_batch < 1 || c.max_batch > 65536 || c.max_inflight < 1 || c.max_inflight > 16 || !(c.hbm_budget_mb > 0) ||
+        !(c.fdm_tol > 0))
+        return drw::set_error(DRW_EINVAL, "server_create: bad configuration");
+    int ndev = 0;
+    DRW_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return drw::set_error(DRW_EINVAL, "server_create: bad device %d", device);
+    DRW_HIP(hipSetDevice(device));
+    auto* sv = new drw_server();
+    sv->device = device;
+    sv->cfg = c;
+    if (models_txt && models_txt[0]) {
+        sv->models = models_txt;
+        sv->nn = true;
+    }
+    sv->slots.resize(c.max_inflight);
+    const size_t B = (size_t)c.max_batch;
+    for (auto& s : sv->slots) {
+        hipError_t e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_L, B * 30 * sizeof(float), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_C, B * 105 * sizeof(float), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_out, B * 5 * sizeof(float), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_ax, B * sizeof(int32_t), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s.h_q, B * sizeof(uint64_t), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_L, B * 30 * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_C, B * 105 * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_out, B * 5 * sizeof(float));
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_ax, B * sizeof(int32_t));
+        if (e == hipSuccess) e = hipMalloc((void**)&s.d_q, B * sizeof(uint64_t));
+        if (e != hipSuccess) {
+            drw_server_destroy(sv);
+            return drw::set_error(DRW_EHIP, "server_create: %s", hipGetErrorString(e));
+        }
+    }
+    // Create the per-slot solvers of the network route up front so that a
+    // bad model list fails here rather than on the first request.
+    if (sv->nn) {
+        for (auto& s : sv->slots)
+            for (int k = 0; k < 2; k++) {
+                drw_solver* h = nullptr;
+                const int rc = sv->slot_solver(s, k * kRoutes, &h);
+                if (rc) {
+                    const std::string msg = drw::last_error();
+                    drw_server_destroy(sv);
+                    return drw::set_error(rc, "%s", msg.c_str());
+                }
+            }
+    }
+    sv->t0 = Clock::now();
+    sv->worker = std::thread([sv] { sv->loop(); });
+    *out = sv;
+    return 0;
+}
+
+int drw_server_submit(drw_server* sv, int kind, int N, int axis, const float* layer, const float* cuboid, uint64_t qid,
+                      uint64_t* ticket)
+{
+    drw::clear_error();
+    if (!sv || !layer || !cuboid || !ticket) return drw::set_error(DRW_EINVAL, "server_submit: null argument");
+    if (kind != DRW_POISSON && kind != DRW_GRADIENT) return drw::set_error(DRW_EINVAL, "server_submit: bad kind");
+    if (N != 16 && N != 21 && N != 23) return drw::set_error(DRW_EINVAL, "server_submit: N must be 16, 21 or 23");
+    if (kind == DRW_GRADIENT && (axis < 0 || axis > 2))
+        return drw::set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    auto q = std::make_unique<Req>();
+    q->kind = kind;
+    q->N = N;
+    q->axis = kind == DRW_GRADIENT ? axis : 2;
+    q->cls = kind * kRoutes + route_of(N, sv->nn);
+    std::memcpy(q->in, layer, sizeof(float) * 30);
+    std::memcpy(q->in + 30, cuboid, sizeof(float) * 105);
+    q->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->stop) return drw::set_error(DRW_ESTATE, "server is shutting down");
+        q->ticket = sv->next_ticket++;
+        q->qid = qid ? qid : q->ticket;
+        *ticket = q->ticket;
+        sv->queue[q->cls].push_back(q.get());
+        sv->reqs[q->ticket] = std::move(q);
+    }
+    sv->cv_work.notify_one();
+    return 0;
+}
+
+int drw_server_wait(drw_server* sv, uint64_t ticket, float* out, int timeout_ms)
+{
+    drw::clear_error();
+    if (!sv || !out) return drw::set_error(DRW_EINVAL, "server_wait: null argument");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto it = sv->reqs.find(ticket);
+    if (it == sv->reqs.end()) return drw::set_error(DRW_EINVAL, "server_wait: unknown ticket %llu", (unsigned long long)ticket);
+    Req* q = it->second.get();
+    auto ready = [&] { return q->done; };
+    if (timeout_ms < 0) sv->cv_done.wait(lk, ready);
+    else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
+        return drw::set_error(DRW_ESTATE, "server_wait: timed out");
+    const int rc = q->rc;
+    std::memcpy(out, q->out, sizeof(float) * (q->kind == DRW_GRADIENT ? 5 : 3));
+    sv->reqs.erase(it);
+    if (rc) return drw::set_error(rc, "server: batch failed");
+    return 0;
+}
+
+int drw_server_stats_get(drw_server* sv, drw_server_stats* st, int reset)
+{
+    if (!sv || !st) return drw::set_error(DRW_EINVAL, "server_stats: null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::memset(st, 0, sizeof *st);
+    st->completed = sv->completed;
+    st->batches = sv->batches;
+    st->mean_batch = sv->batches ? (double)sv->batched / (double)sv->batches : 0.0;
+    st->inflight_peak = sv->inflight_peak;
+    st->elapsed_s = std::chrono::duration<double>(Clock::now() - sv->t0).count();
+    st->qps = st->elapsed_s > 0 ? (double)sv->completed / st->elapsed_s : 0.0;
+    if (!sv->lat_ms.empty()) {
+        std::vector<float> v(sv->lat_ms);
+        auto pct = [&](double p) {
+            const size_t k = std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5));
+            std::nth_element(v.begin(), v.begin() + k, v.end());
+            return (double)v[k];
+        };
+        st->p50_ms = pct(0.50);
+        st->p99_ms = pct(0.99);
+        st->max_ms = *std::max_element(sv->lat_ms.begin(), sv->lat_ms.end());
+    }
+    if (reset) {
+        sv->lat_ms.clear();
+        sv->completed = sv->batches = sv->batched = sv->inflight_peak = 0;
+        sv->t0 = Clock::now();
+    }
+    return 0;
+}
+
+void drw_server_destroy(drw_server* sv)
+{
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();  // drains the queues first
+    (void)hipSetDevice(sv->device);
+    for (auto& s : sv->slots) {
+        if (s.stream) (void)hipStreamSynchronize(s.stream);
+        for (auto* h : s.solvers)
+            if (h) drw_solver_destroy(h);
+        if (s.h_L) (void)hipHostFree(s.h_L);
+        if (s.h_C) (void)hipHostFree(s.h_C);
+        if (s.h_out) (void)hipHostFree(s.h_out);
+        if (s.h_ax) (void)hipHostFree(s.h_ax);
+        if (s.h_q) (void)hipHostFree(s.h_q);
+        if (s.d_L) (void)hipFree(s.d_L);
+        if (s.d_C) (void)hipFree(s.d_C);
+        if (s.d_out) (void)hipFree(s.d_out);
+        if (s.d_ax) (void)hipFree(s.d_ax);
+        if (s.d_q) (void)hipFree(s.d_q);
+        if (s.done) (void)hipEventDestroy(s.done);
+        if (s.stream) (void)hipStreamDestroy(s.stream);
+    }
+    delete sv;
+}
+
+}  // extern "C"

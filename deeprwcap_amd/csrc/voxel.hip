@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void sample_flat_kernel(FlatSampleArgs a)
     const int N = a.N[b], NN = N * N, M = 6 * NN;
     const double* G = a.G + a.G_off[b];
     uint32_t u[4];
-    query_uniforms(a.seed, a.counter + (uint64_t)b, 0u, u);
+    query_uniforms(a.seed, a.qid ? a.qid[b] : a.counter + (uint64_t)b, 0u, u);
     const int i = wave_cdf_draw<double>(G, M, u[0]);
     const bool grad = a.axis != nullptr;
     const int width = grad ? 5 : 3;

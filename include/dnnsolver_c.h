@@ -122,6 +122,12 @@ int drw_solver_sample(drw_solver* s, int B, const float* layers, const float* cu
 int drw_solver_sample_device(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
                              float* out, void* stream);
 
+/* Same with a caller-given Philox counter per query (device array qid[B]):
+ * a query's hop depends only on its own input and counter, not on the batch
+ * it travels in (used by the serving engine). */
+int drw_solver_sample_ids(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                          const uint64_t* qid, float* out, void* stream);
+
 /* Network backend with diagnostics (device pointers, each optional):
  * face_out [B] drawn face, probs_out [B][6] selector probabilities,
  * weight_out [B] predicted W (gradient), kern_out [B][529] normalised kernel
@@ -140,6 +146,40 @@ int drw_solver_sample_ex(drw_solver* s, int B, const float* layers, const float*
  * gradient: G[remap(f)][t] = W * probs[f] * kern[f][t]. */
 int drw_solver_eval_kernels(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
                             float* probs_out, float* weight_out, float* kern_out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Serving engine (replaces ParallelSampler, parallel_sampler.cpp:42-211, for
+ * callers that mix query kinds and resolutions): thread-safe submit/wait,
+ * continuous batching with up to max_inflight batches on the device (one HIP
+ * stream each), admission bounded by an HBM budget, per-request Philox
+ * counters, latency percentiles.  N = 23 queries go to the networks when a
+ * model list is given; other N (16, 21) and model-less servers use the FDM
+ * fallback at the query's N.
+ * ------------------------------------------------------------------------ */
+typedef struct drw_server drw_server;
+typedef struct {
+    int max_batch;         /* queries per launch (default 2048) */
+    int max_inflight;      /* concurrent batches / streams (default 4) */
+    double hbm_budget_mb;  /* device memory for in-flight queries (default 1024) */
+    double fdm_tol;        /* FDM relative residual (default 1e-6) */
+    uint64_t seed;         /* Philox key */
+} drw_server_config;
+typedef struct {
+    uint64_t completed, batches, inflight_peak;
+    double mean_batch, p50_ms, p99_ms, max_ms, elapsed_s, qps;
+} drw_server_stats;
+
+int drw_server_create(int device, const char* models_txt, const drw_server_config* cfg, drw_server** out);
+/* kind DRW_POISSON/DRW_GRADIENT, N in {16, 21, 23}, axis (gradient),
+ * layer[30], cuboid[105]; qid = Philox counter of this query (0: the ticket). */
+int drw_server_submit(drw_server* sv, int kind, int N, int axis, const float* layer, const float* cuboid, uint64_t qid,
+                      uint64_t* ticket);
+/* Blocks until the query is done (timeout_ms < 0: no limit); out receives 3
+ * or 5 floats.  Each ticket can be waited for once. */
+int drw_server_wait(drw_server* sv, uint64_t ticket, float* out, int timeout_ms);
+int drw_server_stats_get(drw_server* sv, drw_server_stats* stats, int reset);
+/* Drains pending queries, then frees everything. */
+void drw_server_destroy(drw_server* sv);
 
 #ifdef __cplusplus
 }

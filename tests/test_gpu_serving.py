@@ -1,0 +1,75 @@
+"""GPU tests of the serving engine: each query's result is bit-identical
+whether it runs alone or in a concurrent mix of kinds and resolutions, and
+the latency statistics are reported."""
+import os
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def models_txt(tmp_path_factory):
+    from deeprwcap_amd import weights as W
+    d = np.load(os.path.join(ROOT, "tests", "golden", "nets_golden.npz"))
+    states = {n: W.state_from_npz(d, n) for n in W.MODEL_ORDER}
+    return W.export_model_dir(str(tmp_path_factory.mktemp("models")), states)
+
+
+def _requests(n, seed):
+    from deeprwcap_amd import structures as S
+    rng = np.random.default_rng(seed)
+    Ns = rng.choice([16, 21, 23], n)
+    L, C, _ = S.layered_boxes(n, Ns, seed=seed)
+    kinds = rng.integers(0, 2, n)
+    axes = rng.integers(0, 3, n)
+    return [(int(kinds[i]), int(Ns[i]), int(axes[i]), L[i], C[i], 5000 + i) for i in range(n)]
+
+
+def test_alone_equals_mix(models_txt):
+    from deeprwcap_amd.serving import Server
+    reqs = _requests(160, 3)
+    sv = Server(0, models_txt, max_batch=64, max_inflight=4, hbm_budget_mb=64)
+    alone = []
+    for k, n, a, L, C, q in reqs:
+        alone.append(sv.wait(sv.submit(k, n, a, L, C, q), k))
+    with ThreadPoolExecutor(8) as ex:
+        tickets = list(ex.map(lambda r: sv.submit(r[0], r[1], r[2], r[3], r[4], r[5]), reqs))
+    mix = [sv.wait(t, r[0]) for t, r in zip(tickets, reqs)]
+    st = sv.stats()
+    sv.close()
+    for i, (a, m) in enumerate(zip(alone, mix)):
+        np.testing.assert_array_equal(a, m, err_msg=f"request {i} {reqs[i][:3]}")
+        assert np.isfinite(a).all()
+    assert st["completed"] == 2 * len(reqs)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    print("serving stats", st, flush=True)
+
+
+def test_server_matches_solver(models_txt):
+    """A served N=23 Poisson query equals the solver's batched result for the
+    same Philox counter."""
+    import torch
+    from deeprwcap_amd import _lib
+    from deeprwcap_amd.dnnsolver import DNNSolverPoisson
+    from deeprwcap_amd.serving import Server
+    from deeprwcap_amd import structures as S
+    B = 48
+    L, C, _ = S.layered_boxes(B, 23, seed=9)
+    qid = np.arange(100, 100 + B, dtype=np.uint64)
+    s = DNNSolverPoisson(0)
+    s.loadModels(models_txt=models_txt)
+    s.set_seed(77, 0)
+    dL, dC, dq = torch.from_numpy(L).cuda(), torch.from_numpy(C).cuda(), torch.from_numpy(qid.view(np.int64)).cuda()
+    out = torch.empty((B, 3), device="cuda")
+    _lib.check(_lib.lib().drw_solver_sample_ids(s.handle, B, dL.data_ptr(), dC.data_ptr(), None, dq.data_ptr(),
+                                                 out.data_ptr(), None), "sample_ids")
+    torch.cuda.synchronize()
+    ref = out.cpu().numpy()
+    sv = Server(0, models_txt, max_batch=16, seed=77)
+    got = [sv.wait(sv.submit(0, 23, 2, L[b], C[b], int(qid[b])), 0) for b in range(B)]
+    sv.close()
+    np.testing.assert_array_equal(np.stack(got), ref)
