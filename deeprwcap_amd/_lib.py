@@ -36,6 +36,10 @@ SIGNATURES = {
     "drw_solver_sample": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_device": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_ids": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "drw_solver_graph_create": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, ctypes.POINTER(_c_vp)]),
+    "drw_graph_launch": (_c_int, [_c_vp, _c_vp]),
+    "drw_graph_counter": (_c_int, [_c_vp, ctypes.POINTER(_c_u64)]),
+    "drw_graph_destroy": (None, [_c_vp]),
     "drw_solver_sample_ex": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp,
                                       _c_vp]),
     "drw_server_create": (_c_int, [_c_int, ctypes.c_char_p, _c_vp, ctypes.POINTER(_c_vp)]),

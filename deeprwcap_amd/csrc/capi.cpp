@@ -115,7 +115,7 @@ int drw_sample_flat(int B, const int32_t* N, const int32_t* axis, const double* 
     clear_error();
     if (B < 0) return set_error(DRW_EINVAL, "drw_sample_flat: B < 0");
     if (!N || !G || !G_off || !out) return set_error(DRW_EINVAL, "drw_sample_flat: null pointer");
-    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out, nullptr};
+    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out, nullptr, nullptr};
     return sample_flat_launch(a, (hipStream_t)stream);
 }
 
@@ -225,7 +225,8 @@ int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int
 }
 
 struct Diag {
-    const uint64_t* qid = nullptr;  // per-query Philox counters (device)
+    const uint64_t* qid = nullptr;      // per-query Philox counters (device)
+    const uint64_t* ctr_dev = nullptr;  // device-resident counter base
     int32_t* face = nullptr;
     float* probs = nullptr;
     float* weight = nullptr;
@@ -258,7 +259,7 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         if ((rc = tm->begin(4, st)) || (rc = fdm_launch(a, ws->fdm_grid, st)) || (rc = tm->end(st))) return rc;
         if (grad && (rc = center_eps_launch(B, s->Nq.p, ax, s->eps.p, s->eps_off.p, s->cen.p, st))) return rc;
         FlatSampleArgs fa{B, s->Nq.p, grad ? ax : nullptr, s->G.p, s->G_off.p, grad ? s->cen.p : nullptr,
-                          s->seed, s->counter, out, dg.qid};
+                          s->seed, s->counter, out, dg.qid, dg.ctr_dev};
         if ((rc = tm->begin(5, st)) || (rc = sample_flat_launch(fa, st)) || (rc = tm->end(st))) return rc;
         s->counter += (uint64_t)B;
         return 0;
@@ -268,6 +269,7 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         q.B = B; q.layers = L; q.cuboids = C; q.axis = grad ? ax : nullptr; q.out = out;
         q.face_out = dg.face; q.probs_out = dg.probs; q.weight_out = dg.weight; q.kern_out = dg.kern;
         q.qid = dg.qid;
+        q.ctr_dev = dg.ctr_dev;
         q.seed = s->seed; q.counter = s->counter; q.timer = &s->timer;
         int rc = query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, st);
         if (rc) return rc;
@@ -336,6 +338,88 @@ int drw_solver_sample_ids(drw_solver* s, int B, const float* layers, const float
                               stream ? (hipStream_t)stream : s->stream, dg);
     s->counter = keep;  // per-query counters given: the solver's own sequence is untouched
     return rc;
+}
+
+struct drw_graph {
+    drw_solver* s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t* d_counter = nullptr;
+    int B = 0;
+};
+
+int drw_solver_graph_create(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                            float* out, drw_graph** g_out)
+{
+    clear_error();
+    if (!s || !g_out || !layers || !cuboids || !out || B <= 0) return set_error(DRW_EINVAL, "graph_create: bad argument");
+    if (s->kind == DRW_GRADIENT && !axis) return set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    if (s->backend < 0) return set_error(DRW_ESTATE, "graph_create: solver has no backend");
+    DRW_HIP(hipSetDevice(s->device));
+    drw_graph* g = new drw_graph();
+    g->s = s;
+    g->B = B;
+    hipError_t e = hipMalloc(&g->d_counter, sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemcpy(g->d_counter, &s->counter, sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete g;
+        return set_error(DRW_EHIP, "graph_create: %s", hipGetErrorString(e));
+    }
+    const int32_t* ax = s->kind == DRW_GRADIENT ? axis : nullptr;
+    // One eager step sizes every buffer (no allocation may happen under
+    // capture).  Replay k then uses query counters c0 + k B, c0 = the
+    // solver's counter now; the solver's own counter is left unchanged.
+    const uint64_t keep = s->counter;
+    Diag dg;
+    dg.ctr_dev = g->d_counter;
+    int rc = solver_run(s, B, layers, cuboids, ax, out, s->stream, dg);
+    s->counter = keep;
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = set_error(DRW_EHIP, "graph_create: warm-up failed");
+    if (!rc) {
+        if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            rc = set_error(DRW_EHIP, "graph_create: begin capture failed");
+        } else {
+            rc = solver_run(s, B, layers, cuboids, ax, out, s->stream, dg);
+            if (!rc) rc = advance_counter_launch(g->d_counter, (uint64_t)B, s->stream);
+            const hipError_t ce = hipStreamEndCapture(s->stream, &g->graph);
+            if (!rc && ce != hipSuccess) rc = set_error(DRW_EHIP, "graph_create: end capture failed: %s", hipGetErrorString(ce));
+        }
+        s->counter = keep;
+        if (!rc && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = set_error(DRW_EHIP, "graph_create: instantiate failed");
+    }
+    if (rc) {
+        drw_graph_destroy(g);
+        return rc;
+    }
+    *g_out = g;
+    return 0;
+}
+
+int drw_graph_launch(drw_graph* g, void* stream)
+{
+    clear_error();
+    if (!g || !g->exec) return set_error(DRW_EINVAL, "graph_launch: null graph");
+    DRW_HIP(hipGraphLaunch(g->exec, stream ? (hipStream_t)stream : g->s->stream));
+    return 0;
+}
+
+int drw_graph_counter(drw_graph* g, uint64_t* counter)
+{
+    clear_error();
+    if (!g || !counter) return set_error(DRW_EINVAL, "graph_counter: null argument");
+    DRW_HIP(hipStreamSynchronize(g->s->stream));
+    DRW_HIP(hipMemcpy(counter, g->d_counter, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+void drw_graph_destroy(drw_graph* g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_counter) (void)hipFree(g->d_counter);
+    delete g;
 }
 
 int drw_solver_eval_kernels(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,

@@ -28,6 +28,7 @@ struct FlatSampleArgs {
     uint64_t seed, counter;    // Philox key and first query counter
     float* out;                // [B, 3] or [B, 5]
     const uint64_t* qid;       // [B] per-query counters (optional; else counter + b)
+    const uint64_t* ctr_dev;   // device-resident counter base (optional; replaces counter)
 };
 int sample_flat_launch(const FlatSampleArgs& a, hipStream_t stream);
 
@@ -37,6 +38,7 @@ int center_eps_launch(int B, const int32_t* N, const int32_t* axis, const float*
 }  // namespace drw
 
 namespace drw {
+int advance_counter_launch(uint64_t* ctr, uint64_t n, hipStream_t stream);
 int fill_meta_launch(int B, int N, const int32_t* axis, int32_t* Nq, int32_t* code, int64_t* eps_off, int64_t* G_off,
                      hipStream_t stream);
 }  // namespace drw

@@ -33,6 +33,7 @@ struct FaceQueryArgs {
     float* kern_out;         // [items, 529] normalised (post-processed) kernels, optional in sample mode
     uint64_t seed, counter;
     const uint64_t* qid;     // [B] per-query counters (optional; else counter + b)
+    const uint64_t* ctr_dev; // device-resident counter base (optional)
 };
 
 // mode 0: PoissonPredictor, 1: Gradient1Predictor, 2: Gradient2Predictor.

@@ -286,7 +286,8 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     // ---- tile draw with the query's second uniform word, then locate
     if (threadIdx.x < 64) {
         uint32_t u[4];
-        query_uniforms(a.seed, a.qid ? a.qid[b] : a.counter + (uint64_t)b, 0u, u);
+        const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
+        query_uniforms(a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, 0u, u);
         const int t = wave_cdf_draw<float>(kern, kNN, u[1]);
         if (threadIdx.x == 0) {
             const int width = MODE == 0 ? 3 : 5;

@@ -333,7 +333,7 @@ int query_nn_launch(const NetWeights* w, const QueryArgs& q, uint8_t** scratch, 
     float* wgt = (float*)(s + off_w);
     int32_t* face = q.face_out ? q.face_out : (int32_t*)(s + off_face);
     SelBackboneArgs ba{B, q.layers, q.cuboids, grad ? q.axis : nullptr, feat};
-    SelHeadArgs ha{B, feat, probs, grad ? wgt : nullptr, q.full ? nullptr : face, q.seed, q.counter, q.qid};
+    SelHeadArgs ha{B, feat, probs, grad ? wgt : nullptr, q.full ? nullptr : face, q.seed, q.counter, q.qid, q.ctr_dev};
     StageTimer* tm = q.timer;
     auto timed = [&](int stage, auto&& fn) -> int {
         int r;
@@ -346,7 +346,7 @@ int query_nn_launch(const NetWeights* w, const QueryArgs& q, uint8_t** scratch, 
     FaceQueryArgs fa{};
     fa.B = B; fa.full = q.full; fa.layers = q.layers; fa.cuboids = q.cuboids; fa.face = face;
     fa.axis = grad ? q.axis : nullptr; fa.weight = wgt; fa.out = q.out; fa.kern_out = q.kern_out;
-    fa.seed = q.seed; fa.counter = q.counter; fa.qid = q.qid;
+    fa.seed = q.seed; fa.counter = q.counter; fa.qid = q.qid; fa.ctr_dev = q.ctr_dev;
     if (!grad) return timed(2, [&] { return face_query_launch(0, fa, w->pred, stream); });
     rc = timed(2, [&] {
         int r = face_query_launch(1, fa, w->g1, stream);

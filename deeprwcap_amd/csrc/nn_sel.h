@@ -30,6 +30,7 @@ struct SelHeadArgs {
     int32_t* face;         // [B] drawn face (optional)
     uint64_t seed, counter;
     const uint64_t* qid;   // [B] per-query counters (optional; else counter + b)
+    const uint64_t* ctr_dev;  // device-resident counter base (optional)
 };
 
 int selector_backbone_launch(bool weighted, const SelBackboneArgs& ba, const SelNetDev& w, hipStream_t stream);

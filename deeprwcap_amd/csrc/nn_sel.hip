@@ -188,7 +188,8 @@ __global__ __launch_bounds__(256) void selector_head_kernel(SelHeadArgs a, SelNe
         if (WEIGHTED && a.weight) a.weight[b] = lg[6];
         if (a.face) {
             uint32_t u[4];
-            query_uniforms(a.seed, a.qid ? a.qid[b] : a.counter + (uint64_t)b, 0u, u);
+            const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
+            query_uniforms(a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, 0u, u);
             const int f = small_cdf_draw(p, 6, u[0]);
             a.face[b] = f < 0 ? 0 : f;
         }

@@ -29,6 +29,7 @@ struct QueryArgs {
     float* kern_out;        // sample: [B, 529] kernel of the drawn face; full: [B, 6, 529]
     uint64_t seed, counter;
     const uint64_t* qid;    // [B] per-query Philox counters (optional; else counter + b)
+    const uint64_t* ctr_dev;  // device-resident counter base (optional; replaces counter)
     StageTimer* timer;      // optional
 };
 

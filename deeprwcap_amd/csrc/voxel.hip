@@ -90,7 +90,8 @@ __global__ __launch_bounds__(256) void sample_flat_kernel(FlatSampleArgs a)
     const int N = a.N[b], NN = N * N, M = 6 * NN;
     const double* G = a.G + a.G_off[b];
     uint32_t u[4];
-    query_uniforms(a.seed, a.qid ? a.qid[b] : a.counter + (uint64_t)b, 0u, u);
+    const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
+    query_uniforms(a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, 0u, u);
     const int i = wave_cdf_draw<double>(G, M, u[0]);
     const bool grad = a.axis != nullptr;
     const int width = grad ? 5 : 3;
@@ -190,6 +191,23 @@ int fill_meta_launch(int B, int N, const int32_t* axis, int32_t* Nq, int32_t* co
     if (B <= 0) return 0;
     hipLaunchKernelGGL(fill_meta_kernel, dim3((B + 255) / 256), dim3(256), 0, stream, B, N, axis, Nq, code, eps_off,
                        G_off);
+    DRW_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace drw
+
+namespace drw {
+
+// Advances a device-resident query counter (the last node of a captured step).
+__global__ void advance_counter_kernel(uint64_t* c, uint64_t n)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c += n;
+}
+
+int advance_counter_launch(uint64_t* ctr, uint64_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(advance_counter_kernel, dim3(1), dim3(64), 0, stream, ctr, n);
     DRW_HIP(hipGetLastError());
     return 0;
 }

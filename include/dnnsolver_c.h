@@ -128,6 +128,20 @@ int drw_solver_sample_device(drw_solver* s, int B, const float* layers, const fl
 int drw_solver_sample_ids(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
                           const uint64_t* qid, float* out, void* stream);
 
+/* hipGraph-captured step: one call records the whole query batch (voxelize,
+ * networks or FDM, draws, locate) on fixed device buffers plus a
+ * device-resident counter advance, so each drw_graph_launch is a single
+ * submission that draws fresh uniforms: launch k uses query counters
+ * c0 + k B with c0 the solver's counter at creation (the solver's own counter
+ * is not advanced; drw_graph_counter reads the graph's).  The graph must be
+ * destroyed before its solver. */
+typedef struct drw_graph drw_graph;
+int drw_solver_graph_create(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
+                            float* out, drw_graph** graph);
+int drw_graph_launch(drw_graph* graph, void* stream);
+int drw_graph_counter(drw_graph* graph, uint64_t* counter);
+void drw_graph_destroy(drw_graph* graph);
+
 /* Network backend with diagnostics (device pointers, each optional):
  * face_out [B] drawn face, probs_out [B][6] selector probabilities,
  * weight_out [B] predicted W (gradient), kern_out [B][529] normalised kernel
