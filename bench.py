@@ -119,6 +119,31 @@ def cpu_baseline(L, C, models_txt, seconds=15.0):
                       f"PoissonPredictor, tile draw, locate) in FP32 PyTorch on {cores} threads, {dt:.1f}s"}
 
 
+def serving_runs(models_txt, rates, duration_s=2.0, threads=64):
+    """Serving loop: 64 host threads issue mixed Poisson/gradient N = 23
+    queries with Poisson arrivals; returns per-rate latency and throughput."""
+    from deeprwcap_amd import structures as S
+    from deeprwcap_amd.serving import Server
+    pool = 4096
+    L, C, _ = S.layered_boxes(pool, 23, seed=99)
+    rng = np.random.default_rng(7)
+    kinds = rng.integers(0, 2, pool).astype(np.int32)
+    axes = rng.integers(0, 3, pool).astype(np.int32)
+    Ns = np.full(pool, 23, np.int32)
+    sv = Server(0, models_txt, max_batch=1024, max_inflight=4, hbm_budget_mb=512)
+    sv.run_load(L, C, kinds, Ns, axes, threads=threads, rate_qps=0, duration_s=0.3)  # warm-up
+    out = []
+    for r in rates:
+        st = sv.run_load(L, C, kinds, Ns, axes, threads=threads, rate_qps=r, duration_s=duration_s,
+                         max_outstanding=256)
+        out.append({"offered_qps": r if r > 0 else None, "achieved_qps": st["qps"], "p50_ms": st["p50_ms"],
+                    "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"], "completed": st["completed"]})
+        log(f"serving offered {r:g}/s: {st['qps']:,.0f}/s p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms "
+            f"batch {st['mean_batch']:.1f}")
+    sv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +153,7 @@ def main():
     ap.add_argument("--models", default=os.path.join(ROOT, "models", "models.txt"))
     ap.add_argument("--fdm-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -221,6 +247,9 @@ def main():
                      "traffic": None,
                      "note": "algorithmic 1,292,876 FLOP per query; most of it is depthwise conv + GELU on the VALU"},
     }
+    if not args.no_serving:
+        res["serving"] = serving_runs(models_txt, [1e5, 1e6, 0])
+        res["p99_latency_ms"] = res["serving"][1]["p99_ms"]
     if not args.no_cpu:
         log("timing the CPU path ...")
         res["cpu_baseline"] = cpu_baseline(L, C, models_txt)

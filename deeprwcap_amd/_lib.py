@@ -47,6 +47,8 @@ SIGNATURES = {
     "drw_server_wait": (_c_int, [_c_vp, _c_u64, _c_vp, _c_int]),
     "drw_server_stats_get": (_c_int, [_c_vp, _c_vp, _c_int]),
     "drw_server_destroy": (None, [_c_vp]),
+    "drw_walk_capacitance": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "drw_server_run_load": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp]),
     "drw_solver_eval_kernels": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
 }
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -403,6 +404,69 @@ int drw_server_stats_get(drw_server* sv, drw_server_stats* st, int reset)
         sv->completed = sv->batches = sv->batched = sv->inflight_peak = 0;
         sv->t0 = Clock::now();
     }
+    return 0;
+}
+
+// Open-loop load: `threads` host threads (the walker threads) each submit
+// queries drawn round-robin from the given pool with exponential
+// inter-arrival times (aggregate rate `rate_qps`; <= 0 means back-to-back),
+// for `duration_s`, then wait for all their tickets.
+int drw_server_run_load(drw_server* sv, const drw_load_config* lc, const float* layers, const float* cuboids,
+                        const int32_t* kinds, const int32_t* Ns, const int32_t* axes, int pool, drw_server_stats* st)
+{
+    drw::clear_error();
+    if (!sv || !lc || !layers || !cuboids || !kinds || !Ns || !axes || pool <= 0 || !st)
+        return drw::set_error(DRW_EINVAL, "server_run_load: bad argument");
+    if (lc->threads < 1 || lc->threads > 1024 || !(lc->duration_s > 0))
+        return drw::set_error(DRW_EINVAL, "server_run_load: bad load configuration");
+    drw_server_stats dummy;
+    drw_server_stats_get(sv, &dummy, 1);
+    std::atomic<int> failed{0};
+    std::atomic<uint64_t> submitted{0};
+    const auto t_end = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                          std::chrono::duration<double>(lc->duration_s));
+    auto worker = [&](int tid) {
+        uint64_t rng = 0x9E3779B97F4A7C15ull * (uint64_t)(tid + 1) ^ lc->seed;
+        auto next_u = [&rng]() {  // xorshift64*
+            rng ^= rng >> 12; rng ^= rng << 25; rng ^= rng >> 27;
+            return (double)((rng * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+        };
+        const double rate = lc->rate_qps > 0 ? lc->rate_qps / lc->threads : 0.0;
+        std::vector<std::pair<uint64_t, int>> mine;
+        auto t_next = Clock::now();
+        int i = tid;
+        while (Clock::now() < t_end) {
+            if (rate > 0) {
+                const double gap = -std::log(1.0 - next_u()) / rate;
+                t_next += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(gap));
+                if (t_next - Clock::now() > std::chrono::microseconds(200)) std::this_thread::sleep_until(t_next);
+                while (Clock::now() < t_next) std::this_thread::yield();
+            }
+            const int k = i % pool;
+            i += lc->threads;
+            uint64_t t = 0;
+            if (drw_server_submit(sv, kinds[k], Ns[k], axes[k], layers + (size_t)k * 30, cuboids + (size_t)k * 105, 0,
+                                  &t)) {
+                failed++;
+                break;
+            }
+            mine.push_back({t, kinds[k]});
+            submitted++;
+            if (mine.size() >= (size_t)lc->max_outstanding) {
+                float o[5];
+                if (drw_server_wait(sv, mine.front().first, o, -1)) failed++;
+                mine.erase(mine.begin());
+            }
+        }
+        float o[5];
+        for (auto& m : mine)
+            if (drw_server_wait(sv, m.first, o, -1)) failed++;
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < lc->threads; t++) th.emplace_back(worker, t);
+    for (auto& t : th) t.join();
+    drw_server_stats_get(sv, st, 0);
+    if (failed) return drw::set_error(DRW_ESTATE, "server_run_load: %d submissions or waits failed", (int)failed);
     return 0;
 }
 

@@ -20,6 +20,23 @@ class ServerStats(ctypes.Structure):
                 ("max_ms", ctypes.c_double), ("elapsed_s", ctypes.c_double), ("qps", ctypes.c_double)]
 
 
+class LoadConfig(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_int), ("max_outstanding", ctypes.c_int), ("rate_qps", ctypes.c_double),
+                ("duration_s", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("window", ctypes.c_double * 6), ("n_layers", ctypes.c_int), ("layer_z", ctypes.c_void_p),
+                ("layer_eps", ctypes.c_void_p), ("n_boxes", ctypes.c_int), ("cond_box", ctypes.c_void_p),
+                ("cond_id", ctypes.c_void_p)]
+
+
+class WalkConfig(ctypes.Structure):
+    _fields_ = [("walks", ctypes.c_int), ("threads", ctypes.c_int), ("concurrent", ctypes.c_int), ("N", ctypes.c_int),
+                ("stop_dist", ctypes.c_double), ("gauss_frac", ctypes.c_double), ("max_cube", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
 class Server:
     """Thread-safe continuous-batching front end for transition queries."""
 
@@ -49,6 +66,44 @@ class Server:
         s = ServerStats()
         check(_lib.lib().drw_server_stats_get(self._h, ctypes.byref(s), int(reset)), "drw_server_stats_get")
         return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def run_load(self, layers, cuboids, kinds, Ns, axes, threads=64, rate_qps=0.0, duration_s=2.0,
+                 max_outstanding=64, seed=1) -> dict:
+        """Open-loop load from `threads` host threads (C++), returns stats."""
+        L = np.ascontiguousarray(layers, np.float32)
+        C = np.ascontiguousarray(cuboids, np.float32)
+        K = np.ascontiguousarray(kinds, np.int32)
+        Nn = np.ascontiguousarray(Ns, np.int32)
+        A = np.ascontiguousarray(axes, np.int32)
+        cfg = LoadConfig(threads, max_outstanding, rate_qps, duration_s, seed)
+        st = ServerStats()
+        check(_lib.lib().drw_server_run_load(self._h, ctypes.byref(cfg), L.ctypes.data, C.ctypes.data, K.ctypes.data,
+                                             Nn.ctypes.data, A.ctypes.data, len(K), ctypes.byref(st)),
+              "drw_server_run_load")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def capacitance(self, geometry: dict, master: int, walks=100000, threads=16, concurrent=64, N=23,
+                    stop_dist=None, gauss_frac=0.5, max_cube=None, seed=1):
+        """FRW capacitance row C[master][:] of a z-layered structure.
+
+        geometry: window (6,), layer_z (n,), layer_eps (n,), boxes (m, 6), ids (m,)."""
+        lz = np.ascontiguousarray(geometry["layer_z"], np.float64)
+        le = np.ascontiguousarray(geometry["layer_eps"], np.float64)
+        bx = np.ascontiguousarray(geometry["boxes"], np.float64)
+        ids = np.ascontiguousarray(geometry["ids"], np.int32)
+        win = np.asarray(geometry["window"], np.float64)
+        scale = float(np.min(bx[:, 3:] - bx[:, :3]))
+        g = Geometry((ctypes.c_double * 6)(*win), len(lz), lz.ctypes.data, le.ctypes.data, len(ids), bx.ctypes.data,
+                     ids.ctypes.data)
+        wc = WalkConfig(walks, threads, concurrent, N, stop_dist or 1e-3 * scale, gauss_frac,
+                        max_cube or float(np.max(win[3:] - win[:3])), seed)
+        nc = int(ids.max()) + 1
+        cap = np.zeros(nc)
+        err = np.zeros(nc)
+        st = np.zeros(4)
+        check(_lib.lib().drw_walk_capacitance(self._h, ctypes.byref(g), master, ctypes.byref(wc), cap.ctypes.data,
+                                              err.ctypes.data, st.ctypes.data), "drw_walk_capacitance")
+        return cap, err, {"walks": st[0], "hops_per_walk": st[1], "gauss_area": st[2], "gauss_margin": st[3]}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -90,3 +90,25 @@ def mixed_N(B: int, seed: int = 0, choices=(16, 21, 23)):
     """N drawn uniformly from ``choices`` (deterministic in seed)."""
     rng = np.random.default_rng(seed ^ 0x5EED)
     return rng.choice(np.asarray(choices, dtype=np.int32), size=B)
+
+
+def wire_crossing(n_wires=8, width=0.1, thick=0.1, length=2.0, pitch=0.2, eps=(3.9, 2.7, 7.0)):
+    """Two orthogonal metal layers of `n_wires` parallel wires (µm): layer 1
+    along x at z in [0.1, 0.2], layer 2 along y at z in [0.3, 0.4]; planar
+    dielectric stack eps[0] below z = 0.25, eps[1] up to z = 0.45, eps[2]
+    above; window with 1 µm clearance.  Returns the geometry dict of
+    Server.capacitance (conductor ids 0..n-1 layer 1, n..2n-1 layer 2)."""
+    span = (n_wires - 1) * pitch + width
+    x0 = (length - span) / 2.0
+    boxes, ids = [], []
+    for i in range(n_wires):
+        y = x0 + i * pitch
+        boxes.append([0.0, y, 0.1, length, y + width, 0.1 + thick])
+        ids.append(i)
+    for i in range(n_wires):
+        x = x0 + i * pitch
+        boxes.append([x, 0.0, 0.3, x + width, length, 0.3 + thick])
+        ids.append(n_wires + i)
+    window = [-1.0, -1.0, -1.0, length + 1.0, length + 1.0, 1.5]
+    return {"window": np.array(window), "layer_z": np.array([-1e9, 0.25, 0.45]), "layer_eps": np.array(eps),
+            "boxes": np.array(boxes), "ids": np.array(ids, np.int32)}

@@ -25,6 +25,7 @@ Module parameter names equal the reference state-dict keys, so the exported
 from __future__ import annotations
 
 import math
+import os
 import time
 
 import numpy as np
@@ -144,7 +145,10 @@ def _kl(pred_logp, target):
     return (target * (t.log() - pred_logp)).sum(-1).mean()
 
 
-def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log=print, seed=0):
+def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log=print, seed=0, minutes=None):
+    """Train one network.  With `minutes`, the run is time-boxed: the number
+    of steps is set from the measured step time so that the cosine schedule
+    ends when the budget does."""
     torch.manual_seed(seed)
     n = cubes.shape[0]
     ntr = int(n * 0.95)
@@ -158,15 +162,11 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
     faces = {"PoissonPredictor": [0, 1, 2, 3, 4, 5], "Gradient1Predictor": [0, 1],
              "Gradient2Predictor": [2, 3, 4, 5]}.get(name, [None])
     opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-10)
-    steps_per_epoch = max(1, (ntr * len(faces)) // batch)
-    total = epochs * steps_per_epoch
-    sched = torch.optim.lr_scheduler.OneCycleLR(opt, max_lr=lr, total_steps=total, pct_start=0.1)
 
     def make_batch(idx, f):
         x = Xall[idx].float()
         if name == "PoissonSelector":
-            tgt = Gp[idx].sum(-1)
-            return x, tgt
+            return x, Gp[idx].sum(-1)
         if name == "GradientSelectorWeight":
             tot = Gg[idx].abs().sum(-1)
             s = tot.sum(-1, keepdim=True)
@@ -184,36 +184,51 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
         if name == "PoissonSelector":
             return _kl(F.log_softmax(out, -1), tgt)
         if name == "GradientSelectorWeight":
-            return _kl(F.log_softmax(out[:, :6], -1), tgt[:, :6]) + 0.1 * F.mse_loss(out[:, 6], tgt[:, 6])
+            p = F.softmax(out[:, :6], -1)
+            return (_kl(F.log_softmax(out[:, :6], -1), tgt[:, :6]) + 0.2 * (p - tgt[:, :6]).abs().sum(-1).mean()
+                    + 0.1 * F.mse_loss(out[:, 6], tgt[:, 6]))
         y = out.reshape(out.shape[0], -1)
         if name == "PoissonPredictor":
             p = F.relu(y) + 1e-10
             p = p / p.sum(-1, keepdim=True)
-            return _kl(p.log(), tgt)
+            return _kl(p.log(), tgt) + 0.2 * (p - tgt).abs().sum(-1).mean()
         p = y / y.abs().sum(-1, keepdim=True)
-        return F.mse_loss(p, tgt) * 529.0 * 50.0
+        return F.mse_loss(p, tgt) * 529.0 * 50.0 + 0.2 * (p - tgt).abs().sum(-1).mean()
 
+    def step(st, total):
+        f = faces[st % len(faces)]
+        idx = torch.randint(0, ntr, (batch,), device=dev)
+        x, tgt = make_batch(idx, f)
+        loss = loss_of(x, tgt)
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
+        warm = max(1, total // 20)
+        frac = st / warm if st < warm else 0.5 * (1 + math.cos(math.pi * (st - warm) / max(1, total - warm)))
+        for gr in opt.param_groups:
+            gr["lr"] = lr * max(frac, 0.02)
+        opt.step()
+        return loss
+
+    model.train()
     t0 = time.time()
-    for ep in range(epochs):
-        model.train()
-        perm = torch.randperm(ntr, device=dev)
-        tot_l, nb = 0.0, 0
-        for st in range(steps_per_epoch):
-            f = faces[st % len(faces)]
-            idx = perm[(st * batch) % ntr:(st * batch) % ntr + batch]
-            if idx.numel() < 2:
-                continue
-            x, tgt = make_batch(idx, f)
-            loss = loss_of(x, tgt)
-            opt.zero_grad(set_to_none=True)
-            loss.backward()
-            torch.nn.utils.clip_grad_norm_(model.parameters(), 1.0)
-            opt.step()
-            sched.step()
+    if minutes is not None:
+        for st in range(20):  # measure the step time
+            step(st, 1000)
+        torch.cuda.synchronize() if dev != "cpu" else None
+        per = (time.time() - t0) / 20
+        total = max(100, int(minutes * 60 / per))
+    else:
+        total = epochs * max(1, (ntr * len(faces)) // batch)
+    tot_l, nb = 0.0, 0
+    for st in range(total):
+        loss = step(st, total)
+        if st % 50 == 0:
             tot_l += loss.item()
             nb += 1
-        if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
-            log(f"{name} epoch {ep + 1}/{epochs} loss {tot_l / max(nb, 1):.5f} ({time.time() - t0:.0f}s)")
+        if st % max(1, total // 10) == 0 or st == total - 1:
+            log(f"{name} step {st + 1}/{total} loss {tot_l / max(nb, 1):.5f} ({time.time() - t0:.0f}s)")
+            tot_l, nb = 0.0, 0
     model.eval()
     return model
 
@@ -241,18 +256,23 @@ def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda"):
     return float(np.mean(errs))
 
 
-def train_all(out_dir, n_cubes=40000, epochs=30, seed=0, log=print, dev="cuda"):
+def train_all(out_dir, n_cubes=40000, epochs=30, seed=0, log=print, dev="cuda", minutes=None, batch=256):
     t0 = time.time()
     cubes, G, Gz = fdm_dataset(n_cubes, seed, dev=dev)
     log(f"dataset: {n_cubes} cubes, FDM kernels in {time.time() - t0:.1f}s")
     states, report = {}, {}
     ho = np.arange(int(n_cubes * 0.95), n_cubes)
+    share = {"PoissonSelector": 0.08, "PoissonPredictor": 0.32, "GradientSelectorWeight": 0.08,
+             "Gradient2Predictor": 0.32, "Gradient1Predictor": 0.20}
     for name in W.MODEL_ORDER:
-        m = train_net(name, cubes, G, Gz, epochs=epochs, dev=dev, log=log)
+        m = train_net(name, cubes, G, Gz, epochs=epochs, dev=dev, log=log, batch=batch,
+                      minutes=None if minutes is None else minutes * share[name])
         states[name] = export_state(m)
         if name.endswith("Predictor"):
             report[name] = evaluate_face_l1(name, m, cubes, G, Gz, ho, dev=dev)
             log(f"{name}: held-out per-face L1 {report[name]:.4f}")
+        os.makedirs(out_dir, exist_ok=True)
+        W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), states[name])  # checkpoint per network
     path = W.export_model_dir(out_dir, states)
     log(f"exported {path} in {time.time() - t0:.0f}s")
     return path, report
@@ -265,9 +285,12 @@ if __name__ == "__main__":
     ap.add_argument("--n", type=int, default=40000)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--minutes", type=float, default=None, help="total wall-clock budget for the five nets")
+    ap.add_argument("--batch", type=int, default=256)
     args = ap.parse_args()
 
     def _log(msg):
         print(msg, flush=True)
 
-    train_all(args.out, n_cubes=args.n, epochs=args.epochs, seed=args.seed, log=_log)
+    train_all(args.out, n_cubes=args.n, epochs=args.epochs, seed=args.seed, log=_log, minutes=args.minutes,
+              batch=args.batch)

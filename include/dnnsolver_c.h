@@ -195,6 +195,55 @@ int drw_server_stats_get(drw_server* sv, drw_server_stats* stats, int reset);
 /* Drains pending queries, then frees everything. */
 void drw_server_destroy(drw_server* sv);
 
+/* Open-loop load generator standing in for the walker threads: `threads` host
+ * threads submit queries from a pool (layers[pool][30], cuboids[pool][105],
+ * kinds/Ns/axes[pool]) with Poisson arrivals at `rate_qps` in total (<= 0:
+ * back-to-back), each keeping at most `max_outstanding` tickets, for
+ * `duration_s`; returns the server statistics of the run (p50/p99 latency is
+ * submit-to-completion). */
+typedef struct {
+    int threads;
+    int max_outstanding;
+    double rate_qps;
+    double duration_s;
+    uint64_t seed;
+} drw_load_config;
+int drw_server_run_load(drw_server* sv, const drw_load_config* load, const float* layers, const float* cuboids,
+                        const int32_t* kinds, const int32_t* Ns, const int32_t* axes, int pool, drw_server_stats* stats);
+
+/* ------------------------------------------------------------------------
+ * Floating-random-walk capacitance extraction (the consumer of the query
+ * path; stands in for the closed walker executable/bin/deepRWCap).  Planar
+ * z-layers of dielectric, conductors as unions of boxes, Dirichlet 0 on the
+ * window.  Walks start on a Gaussian surface around `master`, take one
+ * gradient query then Poisson queries through `sv` until they come within
+ * stop_dist of a conductor; cap[k] receives C[master][k] (the units of the
+ * coordinates times eps; multiply by eps0 for farads), stderr_out[k] its
+ * standard error; stats (optional, 4 doubles): walks, mean hops per walk,
+ * Gaussian-surface area, Gaussian margin.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    double window[6];          /* xmin ymin zmin xmax ymax zmax */
+    int n_layers;
+    const double* layer_z;     /* [n_layers] bottom z of each layer, ascending (first = -inf side) */
+    const double* layer_eps;   /* [n_layers] */
+    int n_boxes;
+    const double* cond_box;    /* [n_boxes][6] */
+    const int32_t* cond_id;    /* [n_boxes] conductor index of each box */
+} drw_geometry;
+typedef struct {
+    int walks;                 /* total walks */
+    int threads;               /* host walker threads */
+    int concurrent;            /* walks in flight per thread */
+    int N;                     /* cube resolution (23: networks when the server has models) */
+    double stop_dist;          /* termination distance to a conductor / the window */
+    double gauss_frac;         /* Gaussian margin as a fraction of the smallest gap */
+    double max_cube;           /* largest cube half side */
+    uint64_t seed;
+} drw_walk_config;
+int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int master, const drw_walk_config* cfg, double* cap,
+                         double* stderr_out, double* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,10 +23,9 @@ def models_txt(tmp_path_factory):
 
 @pytest.fixture(scope="module")
 def params():
-    import torch
     from oracle import nets_ref as R
     d = np.load(os.path.join(ROOT, "tests", "golden", "nets_golden.npz"))
-    return {n: {k: torch.from_numpy(v) for k, v in R.params_from_npz(d, n).items()} for n in R.ARCH}
+    return {n: R.params_from_npz(d, n) for n in R.ARCH}
 
 
 def _solver(kind, models_txt):
