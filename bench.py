@@ -154,6 +154,8 @@ def main():
     ap.add_argument("--fdm-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -165,6 +167,14 @@ def main():
     from deeprwcap_amd.dnnsolver import DNNSolverPoisson
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     B = args.batch
     L, C, _ = S.layered_boxes(B, 23, seed=0)
     dL, dC = torch.from_numpy(L).cuda(), torch.from_numpy(C).cuda()
