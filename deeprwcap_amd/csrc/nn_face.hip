@@ -44,50 +44,83 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int k = 0; k < KT; k++) acc = mfma16(a[k], bw[n][k], acc);
-            const int ch = n * 16 + (lane & 15);
+            constexpr int VFULL = 16;
+            const int V = COUT - n * 16 < VFULL ? COUT - n * 16 : VFULL;  // valid columns of this tile
+            if (V == VFULL) {
+                const int ch = n * 16 + (lane & 15);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int p = m * 16 + kq + i;
-                const float v = ch < COUT ? gelu(acc[i] + bb[n]) : 0.f;
-                out[p * CSO + ch] = (half_t)v;
+                for (int i = 0; i < 4; i++) out[(m * 16 + kq + i) * CSO + ch] = (half_t)gelu(acc[i] + bb[n]);
+            } else {
+                // only V of the 16 columns hold outputs: move them across lanes
+                // (output o = lane*E + e -> row o / V, column o % V, held by lane
+                // col + 16*(row/4) as element row % 4) so each lane evaluates E GELUs
+                const int E = V / 4;
+                for (int e = 0; e < E; e++) {
+                    const int o = lane * E + e, row = o / V, col = o % V;
+                    const int src = col + 16 * (row >> 2), el = row & 3;
+                    float v = 0.f;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const float t = __shfl(acc[i], src, 64);
+                        v = el == i ? t : v;
+                    }
+                    out[(m * 16 + row) * CSO + n * 16 + col] = (half_t)gelu(v + bias[n * 16 + col]);
+                }
             }
         }
     }
 }
 
 // Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU on the
-// VALU: in/out [544][CS], channels 0..C-1, one channel pair per thread.
+// VALU: in/out [544][CS], channels 0..C-1.  A thread owns one (channel pair,
+// column x) and slides down the rows in steps of the dilation, so each
+// output reads 3 new input pairs (the other 6 taps are reused from the
+// previous step) and the column's x-border mask is computed once.
 template <int C, int DIL, int CS>
 __device__ void dw_conv(const half_t* in, half_t* out, const float* w, const float* bias)
 {
     constexpr int CP = C / 2;
-    constexpr int STRIDE = 256 / CP;
-    const int t = threadIdx.x;
-    if (t >= STRIDE * CP) return;
-    const int cp = t % CP;
-    float w0[9], w1[9];
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    for (int item = threadIdx.x; item < CP * kN; item += 256) {
+        const int cp = item % CP, x = item / CP;
+        float w0[9], w1[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) { w0[k] = w[(2 * cp) * 9 + k]; w1[k] = w[(2 * cp + 1) * 9 + k]; }
-    const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
-    for (int p = t / CP; p < kNN; p += STRIDE) {
-        const int y = p / kN, x = p - y * kN;
-        float a0 = b0, a1 = b1;
+        for (int k = 0; k < 9; k++) { w0[k] = w[(2 * cp) * 9 + k]; w1[k] = w[(2 * cp + 1) * 9 + k]; }
+        const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
+        const bool okl = x - DIL >= 0, okr = x + DIL < kN;
+        // three input columns (x-d, x, x+d) of one row, both channels
+        auto load_row = [&](int y, float* v) {
 #pragma unroll
-        for (int ky = 0; ky < 3; ky++) {
-            const int yy = y + DIL * (ky - 1);
-            if (yy < 0 || yy >= kN) continue;
+            for (int q = 0; q < 6; q++) v[q] = 0.f;
+            if (y < 0 || y >= kN) return;
+            const half_t* s = in + (y * kN + x) * CS + 2 * cp;
+            const half2_t c = *(const half2_t*)s;
+            v[2] = (float)c[0]; v[3] = (float)c[1];
+            if (okl) { const half2_t l = *(const half2_t*)(s - DIL * CS); v[0] = (float)l[0]; v[1] = (float)l[1]; }
+            if (okr) { const half2_t r = *(const half2_t*)(s + DIL * CS); v[4] = (float)r[0]; v[5] = (float)r[1]; }
+        };
+#pragma unroll 1
+        for (int r0 = 0; r0 < DIL; r0++) {
+            float up[6], mid[6], dn[6];
+            load_row(r0 - DIL, up);
+            load_row(r0, mid);
+#pragma unroll 1
+            for (int y = r0; y < kN; y += DIL) {
+                load_row(y + DIL, dn);
+                float a0 = b0, a1 = b1;
 #pragma unroll
-            for (int kx = 0; kx < 3; kx++) {
-                const int xx = x + DIL * (kx - 1);
-                if (xx < 0 || xx >= kN) continue;
-                const half_t* s = in + (yy * kN + xx) * CS + 2 * cp;
-                a0 += w0[ky * 3 + kx] * (float)s[0];
-                a1 += w1[ky * 3 + kx] * (float)s[1];
+                for (int kx = 0; kx < 3; kx++) {
+                    a0 += w0[kx] * up[2 * kx] + w0[3 + kx] * mid[2 * kx] + w0[6 + kx] * dn[2 * kx];
+                    a1 += w1[kx] * up[2 * kx + 1] + w1[3 + kx] * mid[2 * kx + 1] + w1[6 + kx] * dn[2 * kx + 1];
+                }
+                half2_t o;
+                o[0] = (half_t)gelu(a0);
+                o[1] = (half_t)gelu(a1);
+                *(half2_t*)(out + (y * kN + x) * CS + 2 * cp) = o;
+#pragma unroll
+                for (int q = 0; q < 6; q++) { up[q] = mid[q]; mid[q] = dn[q]; }
             }
         }
-        half_t* d = out + p * CS + 2 * cp;
-        d[0] = (half_t)gelu(a0);
-        d[1] = (half_t)gelu(a1);
     }
 }
 

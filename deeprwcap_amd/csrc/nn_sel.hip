@@ -19,13 +19,27 @@ namespace drw {
 
 // Conv3d(CIN -> COUT, k3, s2, no padding) over in[CIN][DIN^3] (fp16) as an
 // implicit GEMM; out[COUT][DOUT^3] fp16 after bias + GELU, or fp32 features.
+// koff (LDS, >= KT*16 ints) receives the im2col offset of every k = ci*27 +
+// tap (-1 past K), so a gathered element costs one add and one LDS read.
+// With COUT < 16 only COUT of the 16 MFMA columns hold outputs; the valid
+// ones are first moved across lanes (ds_bpermute) so that every lane
+// evaluates 16*COUT/64 GELUs instead of 4.
 template <int CIN, int COUT, int DIN, int DOUT, bool FP32_OUT>
-__device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, const float* bias)
+__device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, const float* bias, int* koff)
 {
     constexpr int M = DOUT * DOUT * DOUT, MT = (M + 15) / 16;
     constexpr int K = CIN * 27, KT = (K + 15) / 16;
     constexpr int NT = (COUT + 15) / 16;
     constexpr int D2 = DIN * DIN, DO2 = DOUT * DOUT;
+    for (int k = threadIdx.x; k < KT * 16; k += 256) {
+        int o = -1;
+        if (k < K) {
+            const int ci = k / 27, tap = k - 27 * ci;
+            o = ci * DIN * D2 + (tap / 9) * D2 + ((tap / 3) % 3) * DIN + tap % 3;
+        }
+        koff[k] = o;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = 4 * (lane >> 4);
     for (int mt = wave; mt < MT; mt += 4) {
@@ -39,39 +53,56 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
         for (int kt = 0; kt < KT; kt++) {
+            const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
+            const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
             half4_t a;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int k = kt * 16 + kq + j;
-                half_t v = (half_t)0.f;
-                if (base >= 0 && k < K) {
-                    const int ci = k / 27, tap = k - 27 * ci;
-                    const int kz = tap / 9, ky = (tap / 3) % 3, kx = tap % 3;
-                    v = in[ci * DIN * D2 + base + kz * D2 + ky * DIN + kx];
-                }
-                a[j] = v;
-            }
+            for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
 #pragma unroll
             for (int n = 0; n < NT; n++) acc[n] = mfma16(a, frag[(n * KT + kt) * 64 + lane], acc[n]);
         }
+        if constexpr (COUT < 16) {
+            // output o = lane*E + e  ->  (row, col) = (o / COUT, o % COUT), held by
+            // lane col + 16*(row/4) as element row % 4
+            constexpr int E = 16 * COUT / 64;
 #pragma unroll
-        for (int n = 0; n < NT; n++) {
-            const int ch = n * 16 + r;
-            if (ch >= COUT) continue;
-            const float bb = bias[ch];
+            for (int e = 0; e < E; e++) {
+                const int o = lane * E + e, row = o / COUT, col = o % COUT;
+                const int src = col + 16 * (row >> 2), el = row & 3;
+                float v = 0.f;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int mm = mt * 16 + kq + i;
-                if (mm >= M) continue;
-                const float v = gelu(acc[n][i] + bb);
-                if (FP32_OUT) ((float*)outp)[ch * M + mm] = v;
-                else ((half_t*)outp)[ch * M + mm] = (half_t)v;
+                for (int i = 0; i < 4; i++) {
+                    const float t = __shfl(acc[0][i], src, 64);
+                    v = el == i ? t : v;
+                }
+                const int mm = mt * 16 + row;
+                if (mm < M) {
+                    const float g = gelu(v + bias[col]);
+                    if (FP32_OUT) ((float*)outp)[col * M + mm] = g;
+                    else ((half_t*)outp)[col * M + mm] = (half_t)g;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const int ch = n * 16 + r;
+                const float bb = bias[ch];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int mm = mt * 16 + kq + i;
+                    if (mm >= M) continue;
+                    const float v = gelu(acc[n][i] + bb);
+                    if (FP32_OUT) ((float*)outp)[ch * M + mm] = v;
+                    else ((half_t*)outp)[ch * M + mm] = (half_t)v;
+                }
             }
         }
     }
+    __syncthreads();  // koff is rewritten by the next layer
 }
 
 struct SelSmem {
+    int koff[27 * 16];           //  1,728 B  im2col offsets of the current layer
     half_t cube[kNNN];           // 24,334 B
     half_t l1[8 * 1331];         // 21,296 B  (C1 <= 8)
     half_t l2[16 * 125];         //  4,000 B  (C2 <= 16)
@@ -93,11 +124,9 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
         s.cube[v] = (half_t)vox_value(s.desc, z, y, x);
     }
     __syncthreads();
-    conv3d_mfma<1, C1, 23, 11, false>(s.cube, s.l1, w.frag[0], w.bias[0]);
-    __syncthreads();
-    conv3d_mfma<C1, C2, 11, 5, false>(s.l1, s.l2, w.frag[1], w.bias[1]);
-    __syncthreads();
-    conv3d_mfma<C2, C3, 5, 2, true>(s.l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2]);
+    conv3d_mfma<1, C1, 23, 11, false>(s.cube, s.l1, w.frag[0], w.bias[0], s.koff);
+    conv3d_mfma<C1, C2, 11, 5, false>(s.l1, s.l2, w.frag[1], w.bias[1], s.koff);
+    conv3d_mfma<C2, C3, 5, 2, true>(s.l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2], s.koff);
 }
 
 // Linear layer on MFMA for 16 rows: X[16][K] (fp16, LDS) -> Y[16][N].
