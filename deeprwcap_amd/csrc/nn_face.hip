@@ -8,7 +8,8 @@
 // Query flow: dnn_solver_poisson.cpp:86-111, dnn_solver_gradient.cpp:75-164.
 //
 // Activations live in LDS as fp16 [pixel][channel] (pixel = 23 h + w, 529
-// pixels padded to 544 = 34 MFMA row tiles).  1x1 convolutions with >= 8
+// pixels padded to 544 = 34 MFMA row tiles) up to the last MFMA layer and as
+// fp32 in the tail layers (<= 8 channels).  1x1 convolutions with >= 8
 // outputs run on v_mfma_f32_16x16x16f16 (fp32 accumulate); depthwise 3x3,
 // the small 1x1 layers, batch-norm (folded into weights/bias on the host) and
 // GELU run on the VALU in fp32.
@@ -72,15 +73,39 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
 }
 
 // Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU on the
-// VALU: in/out [544][CS], channels 0..C-1.  A thread owns one (channel pair,
+// VALU: in [544][CSI] -> out [544][CSO], channels 0..C-1.  A thread owns one (channel pair,
 // column x) and slides down the rows in steps of the dilation, so each
 // output reads 3 new input pairs (the other 6 taps are reused from the
 // previous step) and the column's x-border mask is computed once.
-template <int C, int DIL, int CS>
-__device__ void dw_conv(const half_t* in, half_t* out, const float* w, const float* bias)
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load_pair(const half_t* s, float& a, float& b)
+{
+    const half2_t c = *(const half2_t*)s;
+    a = (float)c[0];
+    b = (float)c[1];
+}
+__device__ __forceinline__ void load_pair(const float* s, float& a, float& b)
+{
+    const float2 c = *(const float2*)s;
+    a = c.x;
+    b = c.y;
+}
+__device__ __forceinline__ void store_pair(half_t* d, float a, float b)
+{
+    half2_t o;
+    o[0] = (half_t)a;
+    o[1] = (half_t)b;
+    *(half2_t*)d = o;
+}
+__device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float2*)d = make_float2(a, b); }
+
+// The tail layers (8 channels and fewer) keep their activations in fp32: with
+// so few channels per output, fp16 rounding of their inputs would dominate
+// the kernel's error budget, and their storage is small.
+template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
+__device__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias)
 {
     constexpr int CP = C / 2;
-    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
     for (int item = threadIdx.x; item < CP * kN; item += 256) {
         const int cp = item % CP, x = item / CP;
         float w0[9], w1[9];
@@ -93,11 +118,10 @@ __device__ void dw_conv(const half_t* in, half_t* out, const float* w, const flo
 #pragma unroll
             for (int q = 0; q < 6; q++) v[q] = 0.f;
             if (y < 0 || y >= kN) return;
-            const half_t* s = in + (y * kN + x) * CS + 2 * cp;
-            const half2_t c = *(const half2_t*)s;
-            v[2] = (float)c[0]; v[3] = (float)c[1];
-            if (okl) { const half2_t l = *(const half2_t*)(s - DIL * CS); v[0] = (float)l[0]; v[1] = (float)l[1]; }
-            if (okr) { const half2_t r = *(const half2_t*)(s + DIL * CS); v[4] = (float)r[0]; v[5] = (float)r[1]; }
+            const TI* s = in + (y * kN + x) * CSI + 2 * cp;
+            load_pair(s, v[2], v[3]);
+            if (okl) load_pair(s - DIL * CSI, v[0], v[1]);
+            if (okr) load_pair(s + DIL * CSI, v[4], v[5]);
         };
 #pragma unroll 1
         for (int r0 = 0; r0 < DIL; r0++) {
@@ -113,10 +137,7 @@ __device__ void dw_conv(const half_t* in, half_t* out, const float* w, const flo
                     a0 += w0[kx] * up[2 * kx] + w0[3 + kx] * mid[2 * kx] + w0[6 + kx] * dn[2 * kx];
                     a1 += w1[kx] * up[2 * kx + 1] + w1[3 + kx] * mid[2 * kx + 1] + w1[6 + kx] * dn[2 * kx + 1];
                 }
-                half2_t o;
-                o[0] = (half_t)gelu(a0);
-                o[1] = (half_t)gelu(a1);
-                *(half2_t*)(out + (y * kN + x) * CS + 2 * cp) = o;
+                store_pair(out + (y * kN + x) * CSO + 2 * cp, gelu(a0), gelu(a1));
 #pragma unroll
                 for (int q = 0; q < 6; q++) { up[q] = mid[q]; mid[q] = dn[q]; }
             }
@@ -124,9 +145,10 @@ __device__ void dw_conv(const half_t* in, half_t* out, const float* w, const flo
     }
 }
 
-// Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4).
+// Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4), fp32
+// activations.
 template <int CIN, int COUT, int CSI, int CSO>
-__device__ void pw_valu(const half_t* in, half_t* out, const float* w, const float* bias)
+__device__ void pw_valu(const float* in, float* out, const float* w, const float* bias)
 {
     for (int p = threadIdx.x; p < kNN; p += 256) {
         float v[CIN];
@@ -137,19 +159,19 @@ __device__ void pw_valu(const half_t* in, half_t* out, const float* w, const flo
             float a = bias[o];
 #pragma unroll
             for (int c = 0; c < CIN; c++) a += w[o * CIN + c] * v[c];
-            out[p * CSO + o] = (half_t)gelu(a);
+            out[p * CSO + o] = gelu(a);
         }
     }
 }
 
 // Head 1x1 conv CIN -> 1 with bias, no activation: fp32 per pixel.
 template <int CIN, int CSI>
-__device__ void head_conv(const half_t* in, float* hout, const float* w, float b)
+__device__ void head_conv(const float* in, float* hout, const float* w, float b)
 {
     for (int p = threadIdx.x; p < kNN; p += 256) {
         float a = b;
 #pragma unroll
-        for (int c = 0; c < CIN; c++) a += w[c] * (float)in[p * CSI + c];
+        for (int c = 0; c < CIN; c++) a += w[c] * in[p * CSI + c];
         hout[p] = a;
     }
 }
@@ -159,8 +181,8 @@ __device__ void head_conv(const half_t* in, float* hout, const float* w, float b
 // alternate A <-> B.
 template <int BCS>
 struct FaceSmem {
-    half_t A[kPix * 32];
-    half_t B[kPix * BCS];
+    alignas(16) half_t A[kPix * 32];  // 16-byte aligned: written as half8, read as half4 / float2
+    alignas(16) half_t B[kPix * BCS];
     VoxDesc desc;
     float red[8];
     int ibuf[4];
@@ -170,35 +192,40 @@ __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w
 {
     pw_mfma<32, 16, 16, 32, 16>(s.A, s.B, w.frag[0], w.pw_bias[0]);
     __syncthreads();
-    dw_conv<16, 1, 16>(s.B, s.A, w.dw_w[0], w.dw_b[0]); __syncthreads();
+    dw_conv<16, 1, 16, 16>(s.B, s.A, w.dw_w[0], w.dw_b[0]); __syncthreads();
     pw_mfma<16, 16, 16, 16, 16>(s.A, s.B, w.frag[1], w.pw_bias[1]); __syncthreads();
-    dw_conv<16, 1, 16>(s.B, s.A, w.dw_w[1], w.dw_b[1]); __syncthreads();
+    dw_conv<16, 1, 16, 16>(s.B, s.A, w.dw_w[1], w.dw_b[1]); __syncthreads();
     pw_mfma<16, 8, 16, 16, 16>(s.A, s.B, w.frag[2], w.pw_bias[2]); __syncthreads();
-    dw_conv<8, 2, 16>(s.B, s.A, w.dw_w[2], w.dw_b[2]); __syncthreads();
-    pw_valu<8, 4, 16, 16>(s.A, s.B, w.pw_w[3], w.pw_bias[3]); __syncthreads();
-    dw_conv<4, 3, 16>(s.B, s.A, w.dw_w[3], w.dw_b[3]); __syncthreads();
-    pw_valu<4, 2, 16, 16>(s.A, s.B, w.pw_w[4], w.pw_bias[4]); __syncthreads();
-    head_conv<2, 16>(s.B, hout, w.head_w, w.head_b);
+    // fp32 tail: A as float[544][8], B as float[544][4]
+    float* fA = (float*)s.A;
+    float* fB = (float*)s.B;
+    dw_conv<8, 2, 16, 8>(s.B, fA, w.dw_w[2], w.dw_b[2]); __syncthreads();
+    pw_valu<8, 4, 8, 4>(fA, fB, w.pw_w[3], w.pw_bias[3]); __syncthreads();
+    dw_conv<4, 3, 4, 4>(fB, fA, w.dw_w[3], w.dw_b[3]); __syncthreads();
+    pw_valu<4, 2, 4, 2>(fA, fB, w.pw_w[4], w.pw_bias[4]); __syncthreads();
+    head_conv<2, 2>(fB, hout, w.head_w, w.head_b);
 }
 
 __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, float* hout)
 {
     pw_mfma<32, 32, 32, 32, 32>(s.A, s.B, w.frag[0], w.pw_bias[0]); __syncthreads();
-    dw_conv<32, 1, 32>(s.B, s.A, w.dw_w[0], w.dw_b[0]); __syncthreads();
+    dw_conv<32, 1, 32, 32>(s.B, s.A, w.dw_w[0], w.dw_b[0]); __syncthreads();
     pw_mfma<32, 32, 32, 32, 32>(s.A, s.B, w.frag[1], w.pw_bias[1]); __syncthreads();
-    dw_conv<32, 1, 32>(s.B, s.A, w.dw_w[1], w.dw_b[1]); __syncthreads();
+    dw_conv<32, 1, 32, 32>(s.B, s.A, w.dw_w[1], w.dw_b[1]); __syncthreads();
     pw_mfma<32, 24, 32, 32, 32>(s.A, s.B, w.frag[2], w.pw_bias[2]); __syncthreads();
-    dw_conv<24, 1, 32>(s.B, s.A, w.dw_w[2], w.dw_b[2]); __syncthreads();
+    dw_conv<24, 1, 32, 32>(s.B, s.A, w.dw_w[2], w.dw_b[2]); __syncthreads();
     pw_mfma<32, 24, 32, 32, 32>(s.A, s.B, w.frag[3], w.pw_bias[3]); __syncthreads();
-    dw_conv<24, 1, 32>(s.B, s.A, w.dw_w[3], w.dw_b[3]); __syncthreads();
+    dw_conv<24, 1, 32, 32>(s.B, s.A, w.dw_w[3], w.dw_b[3]); __syncthreads();
     pw_mfma<32, 16, 16, 32, 32>(s.A, s.B, w.frag[4], w.pw_bias[4]); __syncthreads();
-    dw_conv<16, 2, 32>(s.B, s.A, w.dw_w[4], w.dw_b[4]); __syncthreads();
+    dw_conv<16, 2, 32, 32>(s.B, s.A, w.dw_w[4], w.dw_b[4]); __syncthreads();
     pw_mfma<16, 16, 16, 32, 32>(s.A, s.B, w.frag[5], w.pw_bias[5]); __syncthreads();
-    dw_conv<16, 2, 32>(s.B, s.A, w.dw_w[5], w.dw_b[5]); __syncthreads();
+    dw_conv<16, 2, 32, 32>(s.B, s.A, w.dw_w[5], w.dw_b[5]); __syncthreads();
     pw_mfma<16, 8, 16, 32, 32>(s.A, s.B, w.frag[6], w.pw_bias[6]); __syncthreads();
-    dw_conv<8, 3, 32>(s.B, s.A, w.dw_w[6], w.dw_b[6]); __syncthreads();
-    pw_valu<8, 4, 32, 32>(s.A, s.B, w.pw_w[7], w.pw_bias[7]); __syncthreads();
-    head_conv<4, 32>(s.B, hout, w.head_w, w.head_b);
+    float* fA = (float*)s.A;
+    float* fB = (float*)s.B;
+    dw_conv<8, 3, 32, 8>(s.B, fA, w.dw_w[6], w.dw_b[6]); __syncthreads();
+    pw_valu<8, 4, 8, 4>(fA, fB, w.pw_w[7], w.pw_bias[7]); __syncthreads();
+    head_conv<4, 4>(fB, hout, w.head_w, w.head_b);
 }
 
 // ------------------------------------------------------------ input tensor
@@ -208,40 +235,54 @@ __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetD
 // channels are y (SignedFacePredictor channel_dim = 1).
 __device__ void build_input(half_t* A, const VoxDesc& d, int mode, int face, int axis)
 {
+    // Source voxel of input element (c, h, w): every remap is a permutation
+    // with flips, so along the 23 channels of a pixel exactly one source axis
+    // varies (by +-1 per channel); the cuboid masks of the two fixed axes are
+    // AND-ed once per pixel.
+    auto src = [&](int c, int h, int w, int& z, int& y, int& x) {
+        if (mode == 2) { z = h; y = c; x = w; } else { z = c; y = h; x = w; }
+        if (mode == 0) {
+            remap_poisson(face, z, y, x);
+        } else {
+            remap_gradient(face, z, y, x);
+            remap_axis(axis, z, y, x);
+        }
+    };
+    typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
     for (int it = threadIdx.x; it < kPix * 4; it += 256) {
         const int p = it >> 2, g = (it & 3) * 8;
-        half_t v[8];
+        half8_t pk;
         if (p >= kNN) {
 #pragma unroll
-            for (int j = 0; j < 8; j++) v[j] = (half_t)0.f;
+            for (int j = 0; j < 8; j++) pk[j] = (half_t)0.f;
         } else {
             const int h = p / kN, w = p - h * kN;
+            int z0, y0, x0, z1, y1, x1;
+            src(0, h, w, z0, y0, x0);
+            src(1, h, w, z1, y1, x1);
+            const uint16_t* vm;
+            int base, dir;
+            uint32_t fixedm;
+            const bool zvar = z1 != z0;
+            if (zvar) { vm = d.bz; base = z0; dir = z1 - z0; fixedm = (uint32_t)d.by[y0] & d.bx[x0]; }
+            else if (y1 != y0) { vm = d.by; base = y0; dir = y1 - y0; fixedm = (uint32_t)d.bz[z0] & d.bx[x0]; }
+            else { vm = d.bx; base = x0; dir = x1 - x0; fixedm = (uint32_t)d.bz[z0] & d.by[y0]; }
 #pragma unroll
             for (int j = 0; j < 8; j++) {
                 const int c = g + j;
                 float val = 0.f;
                 if (c < kN) {
-                    int z, y, x;
-                    if (mode == 2) { z = h; y = c; x = w; } else { z = c; y = h; x = w; }
-                    if (mode == 0) {
-                        remap_poisson(face, z, y, x);
-                    } else {
-                        remap_gradient(face, z, y, x);
-                        remap_axis(axis, z, y, x);
-                    }
-                    val = vox_value(d, z, y, x);
+                    const int k = base + dir * c;
+                    const uint32_t m = fixedm & vm[k];
+                    val = m ? d.cv[__builtin_ctz(m)] : d.lz[zvar ? k : z0];
                 } else if (c == kN) {
                     val = (float)w / (float)(kN - 1);
                 } else if (c == kN + 1) {
                     val = (float)h / (float)(kN - 1);
                 }
-                v[j] = (half_t)val;
+                pk[j] = (half_t)val;
             }
         }
-        typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-        half8_t pk;
-#pragma unroll
-        for (int j = 0; j < 8; j++) pk[j] = v[j];
         *(half8_t*)(A + p * 32 + g) = pk;
     }
 }

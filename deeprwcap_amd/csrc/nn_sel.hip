@@ -102,7 +102,7 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 }
 
 struct SelSmem {
-    int koff[27 * 16];           //  1,728 B  im2col offsets of the current layer
+    alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4)
     half_t cube[kNNN];           // 24,334 B
     half_t l1[8 * 1331];         // 21,296 B  (C1 <= 8)
     half_t l2[16 * 125];         //  4,000 B  (C2 <= 16)

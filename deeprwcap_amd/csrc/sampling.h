@@ -63,6 +63,16 @@ __device__ __forceinline__ uint32_t quant(double a, int s) { return (uint32_t)ld
 __device__ __forceinline__ int frexp_exp(float m) { int e; frexpf(m, &e); return e; }
 __device__ __forceinline__ int frexp_exp(double m) { int e; frexp(m, &e); return e; }
 
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((int)(uint32_t)v, o, 64), hi = __shfl_xor((int)(uint32_t)(v >> 32), o, 64);
+        v += ((uint64_t)hi << 32) | lo;
+    }
+    return v;
+}
+
 // Fixed-point inverse-CDF draw over |w[0..M)| by one wave64 (all lanes call,
 // all lanes get the result).  q_i = trunc(|w_i| 2^s) with the power of two s
 // that puts max|w| in [2^23, 2^24); the prefix sums are exact integers, so the
@@ -70,6 +80,10 @@ __device__ __forceinline__ int frexp_exp(double m) { int e; frexp(m, &e); return
 // of the summation order and bit-identical to the CPU sampler.  Returns -1 if
 // every weight is zero or the largest is not finite.  w may live in any
 // address space (global or LDS).
+//
+// Every pass reads w in rows of 64 consecutive bins, one bin per lane
+// (coalesced): max, total, then an inclusive scan row by row (row sums
+// < 64 * 2^24 fit in 32 bits) that stops at the row holding the drawn bin.
 template <typename T, typename Ptr>
 __device__ int wave_cdf_draw(Ptr w, int M, uint32_t u)
 {
@@ -82,31 +96,26 @@ __device__ int wave_cdf_draw(Ptr w, int M, uint32_t u)
     m = wave_max(m);
     if (!(m > 0) || isinf(m)) return -1;
     const int s = 24 - frexp_exp(m);
-    const int C = (M + 63) / 64;
-    const int i0 = lane * C, i1 = min(i0 + C, M);
-    uint64_t loc = 0;
-    for (int i = i0; i < i1; i++) loc += quant(w[i], s);
-    uint64_t incl = loc;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t o = shfl_up_u64(incl, d);
-        if (lane >= d) incl += o;
-    }
-    const uint64_t total = shfl_u64(incl, 63);
+    uint64_t total = 0;
+    for (int i = lane; i < M; i += 64) total += quant(w[i], s);
+    total = wave_sum_u64(total);
     const uint64_t lo = (uint64_t)u * (total & 0xFFFFFFFFull);
     const uint64_t hi = (uint64_t)u * (total >> 32);
     const uint64_t r = hi + (lo >> 32);  // floor(u * total / 2^32), total < 2^48
-    const uint64_t hit = __ballot(incl > r);
-    const int hl = hit ? __ffsll((unsigned long long)hit) - 1 : 63;
-    int idx = M - 1;
-    if (lane == hl) {
-        uint64_t acc = incl - loc;
-        for (int i = i0; i < i1; i++) {
-            acc += quant(w[i], s);
-            if (acc > r) { idx = i; break; }
+    uint64_t base = 0;  // cumulative sum of the rows before j0
+    for (int j0 = 0; j0 < M; j0 += 64) {
+        const int i = j0 + lane;
+        uint32_t incl = i < M ? quant(w[i], s) : 0u;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t o = __shfl_up((int)incl, d, 64);
+            if (lane >= d) incl += o;
         }
+        const uint64_t hit = __ballot(base + incl > r);
+        if (hit) return j0 + __ffsll((unsigned long long)hit) - 1;
+        base += (uint32_t)__shfl((int)incl, 63, 64);
     }
-    return __shfl(idx, hl, 64);
+    return M - 1;
 }
 
 // Panel centre of tile t on face f (locate_index_simple_kernel,

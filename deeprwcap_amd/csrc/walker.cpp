@@ -21,7 +21,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -131,6 +133,7 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
     std::atomic<long long> hops_total{0}, walks_done{0};
     std::atomic<int> failed{0};
     std::mutex mu;
+    std::string first_bad;  // description of the first non-finite query result
     const int N = wc->N;
 
     auto dist_all = [&](const double* p, int* nearest) {
@@ -213,6 +216,27 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
                 if (!active[i]) continue;
                 Walk& w = ws[i];
                 if (drw_server_wait(sv, tk[i], out, -1)) { failed++; active[i] = 0; continue; }
+                const int nout = w.first ? 5 : 3;
+                bool finite = true;
+                for (int k = 0; k < nout; k++) finite = finite && std::isfinite(out[k]);
+                if (!finite) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_bad.empty()) {
+                        char msg[256];
+                        std::snprintf(msg, sizeof msg,
+                                      "%s query at (%.6g, %.6g, %.6g) half side %.6g axis %d returned (%g, %g, %g%s",
+                                      w.first ? "gradient" : "Poisson", w.p[0], w.p[1], w.p[2], w.h, w.axis,
+                                      (double)out[0], (double)out[1], (double)out[2], w.first ? "" : ")");
+                        first_bad = msg;
+                        if (w.first) {
+                            std::snprintf(msg, sizeof msg, ", %g, %g)", (double)out[3], (double)out[4]);
+                            first_bad += msg;
+                        }
+                    }
+                    failed++;
+                    active[i] = 0;
+                    continue;
+                }
                 if (w.first) {
                     // -A eps (s/h) W sign, applied when the walk ends
                     w.w = -area * (double)out[4] * w.sgn / w.h * (double)out[3];
@@ -230,7 +254,9 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
     std::vector<std::thread> th;
     for (int t = 0; t < wc->threads; t++) th.emplace_back(worker, t);
     for (auto& t : th) t.join();
-    if (failed) return drw::set_error(DRW_ESTATE, "walk: %d queries failed: %s", (int)failed, drw::last_error());
+    if (failed)
+        return drw::set_error(DRW_ESTATE, "walk: %d queries failed: %s", (int)failed,
+                              first_bad.empty() ? drw::last_error() : first_bad.c_str());
     const double M = (double)walks_done.load();
     for (int k = 0; k < g.ncond; k++) {
         const double m = sum[k] / M, v = std::max(sum2[k] / M - m * m, 0.0);

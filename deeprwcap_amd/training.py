@@ -213,11 +213,19 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
     model.train()
     t0 = time.time()
     if minutes is not None:
-        for st in range(20):  # measure the step time
+        # measure the step time after the first steps (library kernel
+        # selection and compilation happen there)
+        for st in range(30):
             step(st, 1000)
-        torch.cuda.synchronize() if dev != "cpu" else None
-        per = (time.time() - t0) / 20
-        total = max(100, int(minutes * 60 / per))
+        if dev != "cpu":
+            torch.cuda.synchronize()
+        t1 = time.time()
+        for st in range(30):
+            step(st, 1000)
+        if dev != "cpu":
+            torch.cuda.synchronize()
+        per = (time.time() - t1) / 30
+        total = max(100, int((minutes * 60 - (time.time() - t0)) / per))
     else:
         total = epochs * max(1, (ntr * len(faces)) // batch)
     tot_l, nb = 0.0, 0

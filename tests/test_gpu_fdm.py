@@ -64,6 +64,34 @@ def test_fdm_mixed_n_vs_oracle(dev, orc):
         assert _relerr(k, Gr) < 1e-10
 
 
+@pytest.mark.parametrize("tol,bound", [(1e-12, 1e-10), (1e-5, 5e-3)])
+def test_fdm_layer_only_cubes(dev, orc, tol, bound):
+    """Cubes described by planar z-layers only, as a walker issues them:
+    homogeneous, an interface on the centre plane, off-grid interfaces;
+    N in {16, 21, 23}, all four right-hand sides, finite and within the bound
+    of the FP64 oracle."""
+    descs = [[(-1.5, 3.9)], [(-1.5, 3.9), (0.0, 2.7)], [(-1.5, 2.7), (0.3, 7.0)],
+             [(-1.5, 3.9), (-0.4, 2.7), (0.55, 7.0)]]
+    Ns, codes, cubes = [], [], []
+    for N in (16, 21, 23):
+        for d in descs:
+            L = np.full((1, 15, 2), -9999.0, np.float32)
+            C = np.full((1, 15, 7), -9999.0, np.float32)
+            for i, (z, e) in enumerate(d):
+                L[0, i] = (z, e)
+            cube = orc.voxelize(L, C, N, normalise=False)[0][0]
+            for code in range(4):
+                Ns.append(N)
+                codes.append(code)
+                cubes.append(cube)
+    pk, G, it, rr = _solve(dev, cubes, np.array(Ns, np.int32), np.array(codes, np.int32), tol)
+    for b in range(len(Ns)):
+        Gr, _, _ = orc.fdm_solve(cubes[b].astype(np.float64), codes[b], tol=1e-14)
+        k = G[pk.G_off[b]:pk.G_off[b] + Gr.size]
+        assert np.isfinite(k).all(), f"case {b} N {Ns[b]} code {codes[b]}: non-finite G (iters {it[b]})"
+        assert _relerr(k, Gr) < bound, f"case {b} N {Ns[b]} code {codes[b]}: err {_relerr(k, Gr):.2e}"
+
+
 def test_fdm_full_batch_invariants(dev):
     """Config: 4096 cubes, N uniform in {16, 21, 23}, layered + 0-2 boxes,
     FP32 PCG + FP64 refinement to 1e-6."""
