@@ -258,7 +258,7 @@ def main():
                      "note": "algorithmic 1,292,876 FLOP per query; most of it is depthwise conv + GELU on the VALU"},
     }
     if not args.no_serving:
-        res["serving"] = serving_runs(models_txt, [1e5, 1e6, 0])
+        res["serving"] = serving_runs(models_txt, [1e5, 1e6, 1e7, 0])
         res["p99_latency_ms"] = res["serving"][1]["p99_ms"]
     if not args.no_cpu:
         log("timing the CPU path ...")

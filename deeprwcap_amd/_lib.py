@@ -45,6 +45,8 @@ SIGNATURES = {
     "drw_server_create": (_c_int, [_c_int, ctypes.c_char_p, _c_vp, ctypes.POINTER(_c_vp)]),
     "drw_server_submit": (_c_int, [_c_vp, _c_int, _c_int, _c_int, _c_vp, _c_vp, _c_u64, ctypes.POINTER(_c_u64)]),
     "drw_server_wait": (_c_int, [_c_vp, _c_u64, _c_vp, _c_int]),
+    "drw_server_submit_many": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "drw_server_wait_many": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_int]),
     "drw_server_stats_get": (_c_int, [_c_vp, _c_vp, _c_int]),
     "drw_server_destroy": (None, [_c_vp]),
     "drw_walk_capacitance": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),

@@ -22,61 +22,66 @@ namespace drw {
 
 // -------------------------------------------------------------- layer bodies
 // 1x1 conv on MFMA: in[544][CSI] (CIN_PAD channels read) -> out[544][CSO].
+// The product is formed transposed, D[channel][pixel] = W x X^T (weights as
+// the A operand, activations as B; the packed fragments are the same
+// registers in either role), so a lane ends with 4 consecutive channels of
+// one pixel and writes them with one 8-byte store.
 template <int CIN_PAD, int COUT, int COUT_PAD, int CSI, int CSO>
 __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const float* bias)
 {
     constexpr int KT = CIN_PAD / 16, NT = COUT_PAD / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, kq = 4 * (lane >> 4);
     half4_t bw[NT][KT];
-    float bb[NT];
+    float bb[NT][4];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
 #pragma unroll
         for (int k = 0; k < KT; k++) bw[n][k] = frag[(n * KT + k) * 64 + lane];
-        bb[n] = bias[n * 16 + (lane & 15)];
+        const int V = COUT - n * 16 < 16 ? COUT - n * 16 : 16;
+        // full tile: channels kq..kq+3; half tile (see below): two channels
+        const int c0 = V == 16 ? kq : ((lane >> 5) ? 2 : 0) + 4 * ((lane >> 4) & 1);
+#pragma unroll
+        for (int i = 0; i < 4; i++) bb[n][i] = (V == 16 || i < 2) ? bias[n * 16 + c0 + i] : 0.f;
     }
-    const int r = lane & 15, kq = 4 * (lane >> 4);
     for (int m = wave; m < kPix / 16; m += 4) {
         half4_t a[KT];
 #pragma unroll
         for (int k = 0; k < KT; k++) a[k] = *(const half4_t*)(in + (m * 16 + r) * CSI + k * 16 + kq);
+        half_t* orow = out + (m * 16 + r) * CSO;
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k = 0; k < KT; k++) acc = mfma16(a[k], bw[n][k], acc);
+            for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
             constexpr int VFULL = 16;
-            const int V = COUT - n * 16 < VFULL ? COUT - n * 16 : VFULL;  // valid columns of this tile
+            const int V = COUT - n * 16 < VFULL ? COUT - n * 16 : VFULL;  // valid channels of this tile
             if (V == VFULL) {
-                const int ch = n * 16 + (lane & 15);
+                half4_t o;
 #pragma unroll
-                for (int i = 0; i < 4; i++) out[(m * 16 + kq + i) * CSO + ch] = (half_t)gelu(acc[i] + bb[n]);
+                for (int i = 0; i < 4; i++) o[i] = (half_t)gelu(acc[i] + bb[n][i]);
+                *(half4_t*)(orow + n * 16 + kq) = o;
             } else {
-                // only V of the 16 columns hold outputs: move them across lanes
-                // (output o = lane*E + e -> row o / V, column o % V, held by lane
-                // col + 16*(row/4) as element row % 4) so each lane evaluates E GELUs
-                const int E = V / 4;
-                for (int e = 0; e < E; e++) {
-                    const int o = lane * E + e, row = o / V, col = o % V;
-                    const int src = col + 16 * (row >> 2), el = row & 3;
-                    float v = 0.f;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const float t = __shfl(acc[i], src, 64);
-                        v = el == i ? t : v;
-                    }
-                    out[(m * 16 + row) * CSO + n * 16 + col] = (half_t)gelu(v + bias[n * 16 + col]);
-                }
+                static_assert(COUT % 16 == 0 || COUT % 16 == 8, "tiles hold 16 or 8 valid channels");
+                // channels 0..7 sit in lanes 0..31 (elements 0..3); lanes 32..63
+                // take elements 2, 3 of lane - 32 so every lane evaluates two GELUs
+                const float t2 = __shfl(acc[2], lane & 31, 64), t3 = __shfl(acc[3], lane & 31, 64);
+                const bool hi = lane >= 32;
+                const int c0 = (hi ? 2 : 0) + 4 * ((lane >> 4) & 1);
+                typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+                h2_t o;
+                o[0] = (half_t)gelu((hi ? t2 : acc[0]) + bb[n][0]);
+                o[1] = (half_t)gelu((hi ? t3 : acc[1]) + bb[n][1]);
+                *(h2_t*)(orow + n * 16 + c0) = o;
             }
         }
     }
 }
 
-// Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU on the
-// VALU: in [544][CSI] -> out [544][CSO], channels 0..C-1.  A thread owns one (channel pair,
-// column x) and slides down the rows in steps of the dilation, so each
-// output reads 3 new input pairs (the other 6 taps are reused from the
-// previous step) and the column's x-border mask is computed once.
+// Activation pairs as fp16 or fp32.  The tail layers (8 channels and fewer)
+// keep their activations in fp32: with so few channels per output, fp16
+// rounding of their inputs would dominate the kernel's error budget, and
+// their storage is small.
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void load_pair(const half_t* s, float& a, float& b)
 {
@@ -99,11 +104,54 @@ __device__ __forceinline__ void store_pair(half_t* d, float a, float b)
 }
 __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float2*)d = make_float2(a, b); }
 
-// The tail layers (8 channels and fewer) keep their activations in fp32: with
-// so few channels per output, fp16 rounding of their inputs would dominate
-// the kernel's error budget, and their storage is small.
+// Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU, one
+// output (pixel, channel pair) per item with all 9 taps read from LDS: every
+// item is independent, so all four waves stay busy and the loads of many
+// items are in flight at once.  Channel pairs run fastest within a wave, so
+// each tap is one contiguous 64-lane read.  Needs 256 % (C/2) == 0 (a
+// thread's channel pair, and so its 18 weights, stays fixed); other channel
+// counts use the sliding form below.
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
-__device__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias)
+__device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, const float* w, const float* bias)
+{
+    constexpr int CP = C / 2;
+    static_assert(256 % CP == 0, "channel pair fixed per thread");
+    const int cp = threadIdx.x % CP;
+    float w0[9], w1[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) { w0[k] = w[(2 * cp) * 9 + k]; w1[k] = w[(2 * cp + 1) * 9 + k]; }
+    const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
+#pragma unroll 2
+    for (int item = threadIdx.x; item < CP * kNN; item += 256) {
+        const int p = item / CP;
+        const int y = p / kN, x = p - y * kN;
+        float a0 = b0, a1 = b1;
+#pragma unroll
+        for (int ky = 0; ky < 3; ky++) {
+            const int yy = y + (ky - 1) * DIL;
+            if (yy < 0 || yy >= kN) continue;
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++) {
+                const int xx = x + (kx - 1) * DIL;
+                if (xx < 0 || xx >= kN) continue;
+                float v0, v1;
+                load_pair(in + (yy * kN + xx) * CSI + 2 * cp, v0, v1);
+                a0 += w0[3 * ky + kx] * v0;
+                a1 += w1[3 * ky + kx] * v1;
+            }
+        }
+        store_pair(out + p * CSO + 2 * cp, gelu(a0), gelu(a1));
+    }
+}
+
+// Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU on the
+// VALU: in [544][CSI] -> out [544][CSO], channels 0..C-1, sliding form: a
+// thread owns one (channel pair, column x) and slides down the rows in steps
+// of the dilation, so each output reads 3 new input pairs (the other 6 taps
+// are reused from the previous step) and the column's x-border mask is
+// computed once.
+template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
+__device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float* bias)
 {
     constexpr int CP = C / 2;
     for (int item = threadIdx.x; item < CP * kN; item += 256) {
@@ -145,6 +193,15 @@ __device__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias
     }
 }
 
+// Depthwise layer: direct form when the channel pair can stay fixed per
+// thread, sliding form otherwise (24 channels).
+template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
+__device__ __forceinline__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias)
+{
+    if constexpr (256 % (C / 2) == 0) dw_conv_direct<C, DIL, CSI, CSO>(in, out, w, bias);
+    else dw_conv_slide<C, DIL, CSI, CSO>(in, out, w, bias);
+}
+
 // Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4), fp32
 // activations.
 template <int CIN, int COUT, int CSI, int CSO>
@@ -176,86 +233,46 @@ __device__ void head_conv(const float* in, float* hout, const float* w, float b)
     }
 }
 
-// ------------------------------------------------------------ architectures
-// Buffer A [544][32] holds the 25-channel input (+7 zero channels); layers
-// alternate A <-> B.
-template <int BCS>
-struct FaceSmem {
-    alignas(16) half_t A[kPix * 32];  // 16-byte aligned: written as half8, read as half4 / float2
-    alignas(16) half_t B[kPix * BCS];
-    VoxDesc desc;
-    float red[8];
-    int ibuf[4];
-};
-
-__device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w, float* hout)
+// ------------------------------------------------------------ input layer
+// First 1x1 conv (25 input channels -> COUT) with its operand built in
+// registers: the MFMA A fragment of lane l is pixel m*16 + l%16, channels
+// kt*16 + 4(l/16) + j, i.e. input element T[c][h][w] of
+// mode 0: rotate_poisson_f(cube); mode 1: rotate_gradient_f(axis_rotate(cube));
+// mode 2: the same, permuted so that channels are y (SignedFacePredictor
+// channel_dim = 1); channels 23, 24 are the w and h position planes, 25..31
+// zero.  Every remap is a permutation with flips, so along the 23 channels of
+// a pixel exactly one source axis varies (by +-1 per channel) and the cuboid
+// masks of the two fixed axes are AND-ed once per pixel.
+template <int MODE, int COUT, int CSO>
+__device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out, const half4_t* frag,
+                              const float* bias)
 {
-    pw_mfma<32, 16, 16, 32, 16>(s.A, s.B, w.frag[0], w.pw_bias[0]);
-    __syncthreads();
-    dw_conv<16, 1, 16, 16>(s.B, s.A, w.dw_w[0], w.dw_b[0]); __syncthreads();
-    pw_mfma<16, 16, 16, 16, 16>(s.A, s.B, w.frag[1], w.pw_bias[1]); __syncthreads();
-    dw_conv<16, 1, 16, 16>(s.B, s.A, w.dw_w[1], w.dw_b[1]); __syncthreads();
-    pw_mfma<16, 8, 16, 16, 16>(s.A, s.B, w.frag[2], w.pw_bias[2]); __syncthreads();
-    // fp32 tail: A as float[544][8], B as float[544][4]
-    float* fA = (float*)s.A;
-    float* fB = (float*)s.B;
-    dw_conv<8, 2, 16, 8>(s.B, fA, w.dw_w[2], w.dw_b[2]); __syncthreads();
-    pw_valu<8, 4, 8, 4>(fA, fB, w.pw_w[3], w.pw_bias[3]); __syncthreads();
-    dw_conv<4, 3, 4, 4>(fB, fA, w.dw_w[3], w.dw_b[3]); __syncthreads();
-    pw_valu<4, 2, 4, 2>(fA, fB, w.pw_w[4], w.pw_bias[4]); __syncthreads();
-    head_conv<2, 2>(fB, hout, w.head_w, w.head_b);
-}
-
-__device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, float* hout)
-{
-    pw_mfma<32, 32, 32, 32, 32>(s.A, s.B, w.frag[0], w.pw_bias[0]); __syncthreads();
-    dw_conv<32, 1, 32, 32>(s.B, s.A, w.dw_w[0], w.dw_b[0]); __syncthreads();
-    pw_mfma<32, 32, 32, 32, 32>(s.A, s.B, w.frag[1], w.pw_bias[1]); __syncthreads();
-    dw_conv<32, 1, 32, 32>(s.B, s.A, w.dw_w[1], w.dw_b[1]); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.A, s.B, w.frag[2], w.pw_bias[2]); __syncthreads();
-    dw_conv<24, 1, 32, 32>(s.B, s.A, w.dw_w[2], w.dw_b[2]); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.A, s.B, w.frag[3], w.pw_bias[3]); __syncthreads();
-    dw_conv<24, 1, 32, 32>(s.B, s.A, w.dw_w[3], w.dw_b[3]); __syncthreads();
-    pw_mfma<32, 16, 16, 32, 32>(s.A, s.B, w.frag[4], w.pw_bias[4]); __syncthreads();
-    dw_conv<16, 2, 32, 32>(s.B, s.A, w.dw_w[4], w.dw_b[4]); __syncthreads();
-    pw_mfma<16, 16, 16, 32, 32>(s.A, s.B, w.frag[5], w.pw_bias[5]); __syncthreads();
-    dw_conv<16, 2, 32, 32>(s.B, s.A, w.dw_w[5], w.dw_b[5]); __syncthreads();
-    pw_mfma<16, 8, 16, 32, 32>(s.A, s.B, w.frag[6], w.pw_bias[6]); __syncthreads();
-    float* fA = (float*)s.A;
-    float* fB = (float*)s.B;
-    dw_conv<8, 3, 32, 8>(s.B, fA, w.dw_w[6], w.dw_b[6]); __syncthreads();
-    pw_valu<8, 4, 8, 4>(fA, fB, w.pw_w[7], w.pw_bias[7]); __syncthreads();
-    head_conv<4, 4>(fB, hout, w.head_w, w.head_b);
-}
-
-// ------------------------------------------------------------ input tensor
-// Network input T[c][h][w] (23 channels + 2 position channels) written to
-// A[p = 23h + w][c]; mode 0: rotate_poisson_f(cube); mode 1:
-// rotate_gradient_f(axis_rotate(cube)); mode 2: the same, permuted so that
-// channels are y (SignedFacePredictor channel_dim = 1).
-__device__ void build_input(half_t* A, const VoxDesc& d, int mode, int face, int axis)
-{
-    // Source voxel of input element (c, h, w): every remap is a permutation
-    // with flips, so along the 23 channels of a pixel exactly one source axis
-    // varies (by +-1 per channel); the cuboid masks of the two fixed axes are
-    // AND-ed once per pixel.
+    static_assert(COUT % 16 == 0, "full MFMA column tiles only");
+    constexpr int KT = 2, NT = COUT / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    half4_t bw[NT][KT];
+    float bb[NT][4];
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+#pragma unroll
+        for (int k = 0; k < KT; k++) bw[n][k] = frag[(n * KT + k) * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < 4; i++) bb[n][i] = bias[n * 16 + 4 * (lane >> 4) + i];
+    }
     auto src = [&](int c, int h, int w, int& z, int& y, int& x) {
-        if (mode == 2) { z = h; y = c; x = w; } else { z = c; y = h; x = w; }
-        if (mode == 0) {
+        if (MODE == 2) { z = h; y = c; x = w; } else { z = c; y = h; x = w; }
+        if (MODE == 0) {
             remap_poisson(face, z, y, x);
         } else {
             remap_gradient(face, z, y, x);
             remap_axis(axis, z, y, x);
         }
     };
-    typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-    for (int it = threadIdx.x; it < kPix * 4; it += 256) {
-        const int p = it >> 2, g = (it & 3) * 8;
-        half8_t pk;
-        if (p >= kNN) {
-#pragma unroll
-            for (int j = 0; j < 8; j++) pk[j] = (half_t)0.f;
-        } else {
+    const int r = lane & 15, kq = 4 * (lane >> 4);
+    for (int m = wave; m < kPix / 16; m += 4) {
+        const int p = m * 16 + r;
+        half4_t a[KT];
+        if (p < kNN) {
             const int h = p / kN, w = p - h * kN;
             int z0, y0, x0, z1, y1, x1;
             src(0, h, w, z0, y0, x0);
@@ -268,23 +285,95 @@ __device__ void build_input(half_t* A, const VoxDesc& d, int mode, int face, int
             else if (y1 != y0) { vm = d.by; base = y0; dir = y1 - y0; fixedm = (uint32_t)d.bz[z0] & d.bx[x0]; }
             else { vm = d.bx; base = x0; dir = x1 - x0; fixedm = (uint32_t)d.bz[z0] & d.by[y0]; }
 #pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int c = g + j;
-                float val = 0.f;
-                if (c < kN) {
-                    const int k = base + dir * c;
-                    const uint32_t m = fixedm & vm[k];
-                    val = m ? d.cv[__builtin_ctz(m)] : d.lz[zvar ? k : z0];
-                } else if (c == kN) {
-                    val = (float)w / (float)(kN - 1);
-                } else if (c == kN + 1) {
-                    val = (float)h / (float)(kN - 1);
+            for (int kt = 0; kt < KT; kt++) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int c = kt * 16 + kq + j;
+                    float val = 0.f;
+                    if (c < kN) {
+                        const int k = base + dir * c;
+                        const uint32_t mm = fixedm & vm[k];
+                        val = mm ? d.cv[__builtin_ctz(mm)] : d.lz[zvar ? k : z0];
+                    } else if (c == kN) {
+                        val = (float)w / (float)(kN - 1);
+                    } else if (c == kN + 1) {
+                        val = (float)h / (float)(kN - 1);
+                    }
+                    a[kt][j] = (half_t)val;
                 }
-                pk[j] = (half_t)val;
             }
+        } else {
+#pragma unroll
+            for (int kt = 0; kt < KT; kt++) a[kt] = (half4_t){0, 0, 0, 0};
         }
-        *(half8_t*)(A + p * 32 + g) = pk;
+        // transposed product (see pw_mfma): 4 consecutive channels of pixel p
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            float4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
+            half4_t o;
+#pragma unroll
+            for (int i = 0; i < 4; i++) o[i] = (half_t)gelu(acc[i] + bb[n][i]);
+            *(half4_t*)(out + p * CSO + n * 16 + kq) = o;
+        }
     }
+}
+
+// ------------------------------------------------------------ architectures
+// Two activation buffers X, Y [544][CS] (fp16).  The first layer reads its
+// input from registers; every later 1x1 layer runs in place (a wave loads
+// the 16 rows of a tile into registers before it writes them, and a row's
+// outputs depend on that row only), every depthwise layer moves X <-> Y.
+// The fp32 tail (<= 8 channels, 32 B per pixel) fits the same buffers.
+template <int CS>
+struct FaceSmem {
+    alignas(16) half_t X[kPix * CS];  // 16-byte aligned: read as half4 / float2
+    alignas(16) half_t Y[kPix * CS];
+    VoxDesc desc;
+    float red[8];
+    int ibuf[4];
+};
+
+// Head output: float[529] in Y.
+template <int MODE>
+__device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w, int face, int axis)
+{
+    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0]); __syncthreads();
+    dw_conv<16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0]); __syncthreads();
+    pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.pw_bias[1]); __syncthreads();
+    dw_conv<16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1]); __syncthreads();
+    pw_mfma<16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.pw_bias[2]); __syncthreads();
+    // fp32 tail
+    float* fX = (float*)s.X;
+    float* fY = (float*)s.Y;
+    dw_conv<8, 2, 16, 8>(s.X, fY, w.dw_w[2], w.dw_b[2]); __syncthreads();
+    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[3], w.pw_bias[3]); __syncthreads();
+    dw_conv<4, 3, 4, 4>(fX, fY, w.dw_w[3], w.dw_b[3]); __syncthreads();
+    pw_valu<4, 2, 4, 2>(fY, fX, w.pw_w[4], w.pw_bias[4]); __syncthreads();
+    head_conv<2, 2>(fX, fY, w.head_w, w.head_b);
+}
+
+__device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, int face, int axis)
+{
+    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0]); __syncthreads();
+    dw_conv<32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0]); __syncthreads();
+    pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.pw_bias[1]); __syncthreads();
+    dw_conv<32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1]); __syncthreads();
+    pw_mfma<32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.pw_bias[2]); __syncthreads();
+    dw_conv<24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2]); __syncthreads();
+    pw_mfma<32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.pw_bias[3]); __syncthreads();
+    dw_conv<24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3]); __syncthreads();
+    pw_mfma<32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.pw_bias[4]); __syncthreads();
+    dw_conv<16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4]); __syncthreads();
+    pw_mfma<16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.pw_bias[5]); __syncthreads();
+    dw_conv<16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5]); __syncthreads();
+    pw_mfma<16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.pw_bias[6]); __syncthreads();
+    float* fX = (float*)s.X;
+    float* fY = (float*)s.Y;
+    dw_conv<8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6]); __syncthreads();
+    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7]); __syncthreads();
+    head_conv<4, 4>(fX, fY, w.head_w, w.head_b);
 }
 
 __device__ __forceinline__ float block_sum256(float v, float* red)
@@ -313,12 +402,9 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     if (MODE == 2 && !(face >= 2 && face <= 5)) return;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
     __syncthreads();
-    build_input(s.A, s.desc, MODE == 0 ? 0 : (MODE == 1 ? 1 : 2), face, axis);
+    if constexpr (MODE == 2) face_solver_plus(s, w, face, axis); else face_solver<MODE>(s, w, face, axis);
     __syncthreads();
-    // head output aliases buffer A (free once the first layer has consumed it)
-    float* hout = (float*)s.A;
-    if constexpr (MODE == 2) face_solver_plus(s, w, hout); else face_solver(s, w, hout);
-    __syncthreads();
+    const float* hout = (const float*)s.Y;
     // ---- output head: relu + 1e-10 then /sum, or x / sum|x| (architecture.py:122-132)
     float loc = 0.f;
     for (int p = threadIdx.x; p < kNN; p += 256) {
@@ -326,8 +412,8 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
         loc += MODE == 0 ? (fmaxf(v, 0.f) + 1e-10f) : fabsf(v);
     }
     const float tot = block_sum256(loc, s.red);
-    // normalised kernel -> buffer B region as fp32 [529], post-processed for gradients
-    float* kern = (float*)s.B;
+    // normalised kernel -> buffer X as fp32 [529], post-processed for gradients
+    float* kern = (float*)s.X;
     for (int p = threadIdx.x; p < kNN; p += 256) {
         float v;
         if (MODE == 0) {

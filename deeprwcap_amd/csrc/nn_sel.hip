@@ -101,11 +101,18 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
     __syncthreads();  // koff is rewritten by the next layer
 }
 
+// LDS of one query, sized per architecture (FaceSelector: C1 = 4, C2 = 8,
+// 36.9 KB; FaceSelectorWeight: C1 = 8, C2 = 16, 47.6 KB).  The second conv
+// output reuses the cube's storage: the cube is dead once the first conv has
+// finished (conv3d_mfma ends with a barrier).
+template <int C1, int C2>
 struct SelSmem {
     alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4)
-    half_t cube[kNNN];           // 24,334 B
-    half_t l1[8 * 1331];         // 21,296 B  (C1 <= 8)
-    half_t l2[16 * 125];         //  4,000 B  (C2 <= 16)
+    union {
+        half_t cube[kNNN];  // 24,334 B
+        half_t l2[C2 * 125];
+    };
+    half_t l1[C1 * 1331];
     VoxDesc desc;
 };
 
@@ -113,7 +120,8 @@ template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs a, SelNetDev w)
 {
     constexpr int C1 = WEIGHTED ? 8 : 4, C2 = WEIGHTED ? 16 : 8, C3 = WEIGHTED ? 64 : 16;
-    __shared__ SelSmem s;
+    static_assert(C2 * 125 <= kNNN, "second conv output must fit the cube's storage");
+    __shared__ SelSmem<C1, C2> s;
     const int b = blockIdx.x;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
     __syncthreads();

@@ -14,17 +14,22 @@
 // Classes: (kind, route) with route = network at N = 23 when models are
 // loaded, else the FDM fallback at the request's N.  The oldest pending
 // request decides which class is launched next.
+//
+// Host path: requests live in a fixed ring indexed by ticket (no per-request
+// allocation); an entry's state is an atomic (free -> claimed -> queued ->
+// done -> free), the class queues, the completion wake-ups and the statistics
+// have separate locks, and callers that issue many queries at once (a walker
+// thread with many walks in flight) submit and wait for them in one call.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cmath>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
 #include <deque>
-#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -41,15 +46,18 @@ namespace {
 
 constexpr int kIn = drw::kMaxLayers * 2 + drw::kMaxCuboids * 7;  // 135 floats
 constexpr int kRoutes = 4;                                        // NN23, FDM16, FDM21, FDM23
+constexpr uint64_t kRing = 1u << 16;  // tickets outstanding (submitted, not yet waited for) at most
+
+enum : int { kFree = 0, kClaimed = 1, kQueued = 2, kDone = 3 };
 
 struct Req {
-    uint64_t ticket = 0, qid = 0;
-    int kind = 0, N = 23, axis = 2, cls = 0;
+    std::atomic<uint64_t> ticket{0};  // ticket holding this entry (0: none)
+    std::atomic<int> state{kFree};
+    uint64_t qid = 0;
+    int kind = 0, N = 23, axis = 2, cls = 0, rc = 0;
     float in[kIn];
     float out[5];
-    int rc = 0;
-    bool done = false;
-    Clock::time_point t_submit, t_done;
+    Clock::time_point t_submit;
 };
 
 struct Slot {
@@ -93,24 +101,33 @@ struct drw_server {
     std::string models;
     bool nn = false;
     std::vector<Slot> slots;
+    std::unique_ptr<Req[]> ring{new Req[kRing]};
+    std::atomic<uint64_t> next_ticket{1};
+    // class queues, admission state, shutdown
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work;
     std::deque<Req*> queue[2 * kRoutes];
-    std::map<uint64_t, std::unique_ptr<Req>> reqs;
-    uint64_t next_ticket = 1;
     size_t bytes_inflight = 0;
     bool stop = false;
-    std::thread worker;
+    // completions
+    std::mutex mu_done;
+    std::condition_variable cv_done;
     // statistics
+    std::mutex mu_stats;
     std::vector<float> lat_ms;
     uint64_t completed = 0, batches = 0, batched = 0, inflight_peak = 0;
     Clock::time_point t0 = Clock::now();
-    std::string fatal;
+    std::thread worker;
 
     int slot_solver(Slot& s, int cls, drw_solver** out);
     int launch(Slot& s, int cls, std::vector<Req*>& batch);
     void reap(Slot& s);
     void loop();
+    int submit(int n, const int32_t* kinds, const int32_t* Ns, const int32_t* axes, const float* layers,
+               const float* cuboids, const uint64_t* qids, uint64_t* tickets, const Clock::time_point* t_arrival);
+    Req* lookup(uint64_t ticket);
+    int take(uint64_t ticket, float* out, int timeout_ms);
+    int try_take(uint64_t ticket, float* out);
 };
 
 int drw_server::slot_solver(Slot& s, int cls, drw_solver** out)
@@ -162,23 +179,33 @@ void drw_server::reap(Slot& s)
 {
     const int w = s.cls / kRoutes == DRW_GRADIENT ? 5 : 3;
     const auto now = Clock::now();
-    std::lock_guard<std::mutex> lk(mu);
-    for (size_t i = 0; i < s.batch.size(); i++) {
+    const size_t n = s.batch.size();
+    for (size_t i = 0; i < n; i++) {
         Req* q = s.batch[i];
         if (s.launch_rc == 0) std::memcpy(q->out, s.h_out + i * w, sizeof(float) * w);
         q->rc = s.launch_rc;
-        q->done = true;
-        q->t_done = now;
-        const float ms = std::chrono::duration<float, std::milli>(now - q->t_submit).count();
-        if (lat_ms.size() < (1u << 22)) lat_ms.push_back(ms);
     }
-    completed += s.batch.size();
-    batches += 1;
-    batched += s.batch.size();
-    bytes_inflight -= s.bytes;
+    {
+        std::lock_guard<std::mutex> lk(mu_stats);
+        for (size_t i = 0; i < n && lat_ms.size() < (1u << 22); i++)
+            lat_ms.push_back(std::chrono::duration<float, std::milli>(now - s.batch[i]->t_submit).count());
+        completed += n;
+        batches += 1;
+        batched += n;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        bytes_inflight -= s.bytes;
+    }
+    {
+        // the waiter frees an entry once it sees kDone: nothing here touches
+        // a request after its state store
+        std::lock_guard<std::mutex> lk(mu_done);
+        for (size_t i = 0; i < n; i++) s.batch[i]->state.store(kDone, std::memory_order_release);
+    }
+    cv_done.notify_all();
     s.batch.clear();
     s.busy = false;
-    cv_done.notify_all();
 }
 
 void drw_server::loop()
@@ -206,7 +233,6 @@ void drw_server::loop()
         //    free slots, the batch cap and the HBM budget
         for (auto& s : slots) {
             if (s.busy) continue;
-            std::vector<Req*> batch;
             int cls = -1;
             {
                 std::unique_lock<std::mutex> lk(mu);
@@ -223,14 +249,11 @@ void drw_server::loop()
                 if (bytes_inflight == 0 && room == 0) room = 1;  // always make progress
                 const size_t n = std::min<size_t>({queue[cls].size(), (size_t)cfg.max_batch, room});
                 if (n == 0) break;  // wait for a batch to finish and free memory
-                for (size_t i = 0; i < n; i++) {
-                    batch.push_back(queue[cls].front());
-                    queue[cls].pop_front();
-                }
+                s.batch.assign(queue[cls].begin(), queue[cls].begin() + (ptrdiff_t)n);
+                queue[cls].erase(queue[cls].begin(), queue[cls].begin() + (ptrdiff_t)n);
                 s.bytes = n * per;
                 bytes_inflight += s.bytes;
             }
-            s.batch = batch;
             s.cls = cls;
             s.busy = true;
             s.launch_rc = launch(s, cls, s.batch);
@@ -238,7 +261,7 @@ void drw_server::loop()
             progressed = true;
         }
         {
-            std::lock_guard<std::mutex> lk(mu);
+            std::lock_guard<std::mutex> lk(mu_stats);
             inflight_peak = std::max<uint64_t>(inflight_peak, (uint64_t)busy);
         }
         if (progressed) continue;
@@ -258,6 +281,106 @@ void drw_server::loop()
             });
         }
     }
+}
+
+int drw_server::submit(int n, const int32_t* kinds, const int32_t* Ns, const int32_t* axes, const float* layers,
+                       const float* cuboids, const uint64_t* qids, uint64_t* tickets,
+                       const Clock::time_point* t_arrival)
+{
+    for (int i = 0; i < n; i++) {
+        if (kinds[i] != DRW_POISSON && kinds[i] != DRW_GRADIENT)
+            return drw::set_error(DRW_EINVAL, "server_submit: bad kind (query %d)", i);
+        if (Ns[i] != 16 && Ns[i] != 21 && Ns[i] != 23)
+            return drw::set_error(DRW_EINVAL, "server_submit: N must be 16, 21 or 23 (query %d)", i);
+        if (kinds[i] == DRW_GRADIENT && (!axes || axes[i] < 0 || axes[i] > 2))
+            return drw::set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
+    }
+    std::vector<Req*> mine((size_t)n);
+    const uint64_t t0 = next_ticket.fetch_add((uint64_t)n);
+    const auto now = Clock::now();
+    for (int i = 0; i < n; i++) {
+        const uint64_t t = t0 + (uint64_t)i;
+        Req& q = ring[t & (kRing - 1)];
+        int expect = kFree;
+        if (!q.state.compare_exchange_strong(expect, kClaimed, std::memory_order_acquire)) {
+            for (int j = 0; j < i; j++) {
+                mine[j]->ticket.store(0, std::memory_order_relaxed);
+                mine[j]->state.store(kFree, std::memory_order_release);
+            }
+            return drw::set_error(DRW_ESTATE, "server_submit: more than %llu queries outstanding (wait for tickets)",
+                                  (unsigned long long)kRing);
+        }
+        q.ticket.store(t, std::memory_order_relaxed);
+        q.kind = kinds[i];
+        q.N = Ns[i];
+        q.axis = kinds[i] == DRW_GRADIENT ? axes[i] : 2;
+        q.cls = kinds[i] * kRoutes + route_of(Ns[i], nn);
+        q.qid = qids && qids[i] ? qids[i] : t;
+        q.rc = 0;
+        std::memcpy(q.in, layers + (size_t)i * 30, sizeof(float) * 30);
+        std::memcpy(q.in + 30, cuboids + (size_t)i * 105, sizeof(float) * 105);
+        q.t_submit = t_arrival ? t_arrival[i] : now;
+        mine[i] = &q;
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        if (stop) {
+            for (auto* q : mine) {
+                q->ticket.store(0, std::memory_order_relaxed);
+                q->state.store(kFree, std::memory_order_release);
+            }
+            return drw::set_error(DRW_ESTATE, "server is shutting down");
+        }
+        for (auto* q : mine) {
+            q->state.store(kQueued, std::memory_order_relaxed);
+            queue[q->cls].push_back(q);
+        }
+    }
+    cv_work.notify_one();
+    for (int i = 0; i < n; i++) tickets[i] = t0 + (uint64_t)i;
+    return 0;
+}
+
+Req* drw_server::lookup(uint64_t ticket)
+{
+    if (ticket == 0) return nullptr;
+    Req* q = &ring[ticket & (kRing - 1)];
+    if (q->ticket.load(std::memory_order_acquire) != ticket) return nullptr;
+    const int st = q->state.load(std::memory_order_acquire);
+    return st == kQueued || st == kDone ? q : nullptr;
+}
+
+// Result of a finished request; frees its entry.
+static int consume(Req* q, float* out)
+{
+    std::memcpy(out, q->out, sizeof(float) * (q->kind == DRW_GRADIENT ? 5 : 3));
+    const int rc = q->rc;
+    q->ticket.store(0, std::memory_order_relaxed);
+    q->state.store(kFree, std::memory_order_release);
+    return rc ? drw::set_error(rc, "server: batch failed") : 0;
+}
+
+int drw_server::take(uint64_t ticket, float* out, int timeout_ms)
+{
+    Req* q = lookup(ticket);
+    if (!q) return drw::set_error(DRW_EINVAL, "server_wait: unknown ticket %llu", (unsigned long long)ticket);
+    auto done = [q] { return q->state.load(std::memory_order_acquire) == kDone; };
+    for (int i = 0; i < 64 && !done(); i++) std::this_thread::yield();
+    if (!done()) {
+        std::unique_lock<std::mutex> lk(mu_done);
+        if (timeout_ms < 0) cv_done.wait(lk, done);
+        else if (!cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+            return drw::set_error(DRW_ETIMEOUT, "server_wait: timed out");
+    }
+    return consume(q, out);
+}
+
+int drw_server::try_take(uint64_t ticket, float* out)
+{
+    Req* q = lookup(ticket);
+    if (!q) return drw::set_error(DRW_EINVAL, "server_wait: unknown ticket %llu", (unsigned long long)ticket);
+    if (q->state.load(std::memory_order_acquire) != kDone) return 1;
+    return consume(q, out) ? -1 : 0;
 }
 
 extern "C" {
@@ -333,54 +456,51 @@ int drw_server_submit(drw_server* sv, int kind, int N, int axis, const float* la
 {
     drw::clear_error();
     if (!sv || !layer || !cuboid || !ticket) return drw::set_error(DRW_EINVAL, "server_submit: null argument");
-    if (kind != DRW_POISSON && kind != DRW_GRADIENT) return drw::set_error(DRW_EINVAL, "server_submit: bad kind");
-    if (N != 16 && N != 21 && N != 23) return drw::set_error(DRW_EINVAL, "server_submit: N must be 16, 21 or 23");
-    if (kind == DRW_GRADIENT && (axis < 0 || axis > 2))
-        return drw::set_error(DRW_EINVAL, "Gradient solver requires axis parameter");
-    auto q = std::make_unique<Req>();
-    q->kind = kind;
-    q->N = N;
-    q->axis = kind == DRW_GRADIENT ? axis : 2;
-    q->cls = kind * kRoutes + route_of(N, sv->nn);
-    std::memcpy(q->in, layer, sizeof(float) * 30);
-    std::memcpy(q->in + 30, cuboid, sizeof(float) * 105);
-    q->t_submit = Clock::now();
-    {
-        std::lock_guard<std::mutex> lk(sv->mu);
-        if (sv->stop) return drw::set_error(DRW_ESTATE, "server is shutting down");
-        q->ticket = sv->next_ticket++;
-        q->qid = qid ? qid : q->ticket;
-        *ticket = q->ticket;
-        sv->queue[q->cls].push_back(q.get());
-        sv->reqs[q->ticket] = std::move(q);
-    }
-    sv->cv_work.notify_one();
-    return 0;
+    const int32_t k = kind, n = N, a = axis;
+    return sv->submit(1, &k, &n, &a, layer, cuboid, &qid, ticket, nullptr);
+}
+
+int drw_server_submit_many(drw_server* sv, int n, const int32_t* kinds, const int32_t* Ns, const int32_t* axes,
+                           const float* layers, const float* cuboids, const uint64_t* qids, uint64_t* tickets)
+{
+    drw::clear_error();
+    if (!sv || n < 0 || (n > 0 && (!kinds || !Ns || !layers || !cuboids || !tickets)))
+        return drw::set_error(DRW_EINVAL, "server_submit_many: bad argument");
+    if ((uint64_t)n > kRing) return drw::set_error(DRW_EINVAL, "server_submit_many: n above %llu", (unsigned long long)kRing);
+    if (n == 0) return 0;
+    return sv->submit(n, kinds, Ns, axes, layers, cuboids, qids, tickets, nullptr);
 }
 
 int drw_server_wait(drw_server* sv, uint64_t ticket, float* out, int timeout_ms)
 {
     drw::clear_error();
     if (!sv || !out) return drw::set_error(DRW_EINVAL, "server_wait: null argument");
-    std::unique_lock<std::mutex> lk(sv->mu);
-    auto it = sv->reqs.find(ticket);
-    if (it == sv->reqs.end()) return drw::set_error(DRW_EINVAL, "server_wait: unknown ticket %llu", (unsigned long long)ticket);
-    Req* q = it->second.get();
-    auto ready = [&] { return q->done; };
-    if (timeout_ms < 0) sv->cv_done.wait(lk, ready);
-    else if (!sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready))
-        return drw::set_error(DRW_ESTATE, "server_wait: timed out");
-    const int rc = q->rc;
-    std::memcpy(out, q->out, sizeof(float) * (q->kind == DRW_GRADIENT ? 5 : 3));
-    sv->reqs.erase(it);
-    if (rc) return drw::set_error(rc, "server: batch failed");
-    return 0;
+    return sv->take(ticket, out, timeout_ms);
+}
+
+int drw_server_wait_many(drw_server* sv, int n, const uint64_t* tickets, float* outs, int timeout_ms)
+{
+    drw::clear_error();
+    if (!sv || n < 0 || (n > 0 && (!tickets || !outs))) return drw::set_error(DRW_EINVAL, "server_wait_many: bad argument");
+    const auto deadline = Clock::now() + std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms);
+    int first = 0;
+    for (int i = 0; i < n; i++) {
+        int left = -1;
+        if (timeout_ms >= 0) {
+            const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(deadline - Clock::now()).count();
+            left = (int)std::max<long long>(0, ms);
+        }
+        const int rc = sv->take(tickets[i], outs + (size_t)i * 5, left);
+        if (rc == DRW_ETIMEOUT) return rc;  // this ticket and the later ones stay valid
+        if (rc && !first) first = rc;
+    }
+    return first ? drw::set_error(first, "server_wait_many: %s", drw::last_error()) : 0;
 }
 
 int drw_server_stats_get(drw_server* sv, drw_server_stats* st, int reset)
 {
     if (!sv || !st) return drw::set_error(DRW_EINVAL, "server_stats: null argument");
-    std::lock_guard<std::mutex> lk(sv->mu);
+    std::lock_guard<std::mutex> lk(sv->mu_stats);
     std::memset(st, 0, sizeof *st);
     st->completed = sv->completed;
     st->batches = sv->batches;
@@ -407,22 +527,25 @@ int drw_server_stats_get(drw_server* sv, drw_server_stats* st, int reset)
     return 0;
 }
 
-// Open-loop load: `threads` host threads (the walker threads) each submit
-// queries drawn round-robin from the given pool with exponential
-// inter-arrival times (aggregate rate `rate_qps`; <= 0 means back-to-back),
-// for `duration_s`, then wait for all their tickets.
+// Open-loop load: `threads` host threads (the walker threads) each draw
+// exponential inter-arrival times (aggregate rate `rate_qps`; <= 0 means
+// back-to-back), submit every query that is due in one call, with its
+// arrival time as the start of its latency, collect finished tickets oldest
+// first, and block only when `max_outstanding` tickets are in flight.
 int drw_server_run_load(drw_server* sv, const drw_load_config* lc, const float* layers, const float* cuboids,
                         const int32_t* kinds, const int32_t* Ns, const int32_t* axes, int pool, drw_server_stats* st)
 {
     drw::clear_error();
     if (!sv || !lc || !layers || !cuboids || !kinds || !Ns || !axes || pool <= 0 || !st)
         return drw::set_error(DRW_EINVAL, "server_run_load: bad argument");
-    if (lc->threads < 1 || lc->threads > 1024 || !(lc->duration_s > 0))
+    if (lc->threads < 1 || lc->threads > 1024 || !(lc->duration_s > 0) || lc->max_outstanding < 1 ||
+        (uint64_t)lc->threads * (uint64_t)lc->max_outstanding > kRing / 2)
         return drw::set_error(DRW_EINVAL, "server_run_load: bad load configuration");
     drw_server_stats dummy;
     drw_server_stats_get(sv, &dummy, 1);
     std::atomic<int> failed{0};
-    std::atomic<uint64_t> submitted{0};
+    std::string first_error;
+    std::mutex emu;
     const auto t_end = Clock::now() + std::chrono::duration_cast<Clock::duration>(
                                           std::chrono::duration<double>(lc->duration_s));
     auto worker = [&](int tid) {
@@ -431,42 +554,75 @@ int drw_server_run_load(drw_server* sv, const drw_load_config* lc, const float* 
             rng ^= rng >> 12; rng ^= rng << 25; rng ^= rng >> 27;
             return (double)((rng * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
         };
+        auto fail = [&]() {
+            failed++;
+            std::lock_guard<std::mutex> lk(emu);
+            if (first_error.empty()) first_error = drw::last_error();
+        };
         const double rate = lc->rate_qps > 0 ? lc->rate_qps / lc->threads : 0.0;
-        std::vector<std::pair<uint64_t, int>> mine;
+        const size_t maxo = (size_t)lc->max_outstanding;
+        std::deque<uint64_t> mine;
+        std::vector<int32_t> kk, nn, aa;
+        std::vector<float> LL, CC;
+        std::vector<Clock::time_point> ta;
+        std::vector<uint64_t> tk;
+        float o[5];
         auto t_next = Clock::now();
         int i = tid;
-        while (Clock::now() < t_end) {
-            if (rate > 0) {
-                const double gap = -std::log(1.0 - next_u()) / rate;
-                t_next += std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(gap));
-                if (t_next - Clock::now() > std::chrono::microseconds(200)) std::this_thread::sleep_until(t_next);
-                while (Clock::now() < t_next) std::this_thread::yield();
+        for (;;) {
+            const auto now = Clock::now();
+            if (now >= t_end) break;
+            // 1) every arrival that is due, in one submission
+            kk.clear(); nn.clear(); aa.clear(); LL.clear(); CC.clear(); ta.clear();
+            while (mine.size() + kk.size() < maxo && kk.size() < 512 && (rate <= 0 || t_next <= now)) {
+                const int k = i % pool;
+                i += lc->threads;
+                kk.push_back(kinds[k]);
+                nn.push_back(Ns[k]);
+                aa.push_back(axes[k]);
+                LL.insert(LL.end(), layers + (size_t)k * 30, layers + (size_t)k * 30 + 30);
+                CC.insert(CC.end(), cuboids + (size_t)k * 105, cuboids + (size_t)k * 105 + 105);
+                ta.push_back(rate > 0 ? t_next : now);
+                if (rate > 0)
+                    t_next += std::chrono::duration_cast<Clock::duration>(
+                        std::chrono::duration<double>(-std::log(1.0 - next_u()) / rate));
             }
-            const int k = i % pool;
-            i += lc->threads;
-            uint64_t t = 0;
-            if (drw_server_submit(sv, kinds[k], Ns[k], axes[k], layers + (size_t)k * 30, cuboids + (size_t)k * 105, 0,
-                                  &t)) {
-                failed++;
-                break;
+            if (!kk.empty()) {
+                tk.resize(kk.size());
+                if (sv->submit((int)kk.size(), kk.data(), nn.data(), aa.data(), LL.data(), CC.data(), nullptr,
+                               tk.data(), ta.data())) {
+                    fail();
+                    break;
+                }
+                mine.insert(mine.end(), tk.begin(), tk.end());
             }
-            mine.push_back({t, kinds[k]});
-            submitted++;
-            if (mine.size() >= (size_t)lc->max_outstanding) {
-                float o[5];
-                if (drw_server_wait(sv, mine.front().first, o, -1)) failed++;
-                mine.erase(mine.begin());
+            // 2) finished tickets, oldest first
+            while (!mine.empty()) {
+                const int r = sv->try_take(mine.front(), o);
+                if (r == 1) break;
+                if (r < 0) fail();
+                mine.pop_front();
+            }
+            // 3) block at the outstanding limit, else pace to the next arrival
+            if (mine.size() >= maxo) {
+                if (sv->take(mine.front(), o, -1)) fail();
+                mine.pop_front();
+            } else if (rate > 0) {
+                const auto wake = std::min(t_next, Clock::now() + std::chrono::microseconds(100));
+                if (wake - Clock::now() > std::chrono::microseconds(60)) std::this_thread::sleep_until(wake);
+                else std::this_thread::yield();
             }
         }
-        float o[5];
-        for (auto& m : mine)
-            if (drw_server_wait(sv, m.first, o, -1)) failed++;
+        for (auto t : mine)
+            if (sv->take(t, o, -1)) fail();
     };
     std::vector<std::thread> th;
     for (int t = 0; t < lc->threads; t++) th.emplace_back(worker, t);
     for (auto& t : th) t.join();
     drw_server_stats_get(sv, st, 0);
-    if (failed) return drw::set_error(DRW_ESTATE, "server_run_load: %d submissions or waits failed", (int)failed);
+    if (failed)
+        return drw::set_error(DRW_ESTATE, "server_run_load: %d submissions or waits failed: %s", (int)failed,
+                              first_error.c_str());
     return 0;
 }
 

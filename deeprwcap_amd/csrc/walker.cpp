@@ -179,16 +179,20 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
         std::vector<Walk> ws(wc->concurrent);
         std::vector<char> active(wc->concurrent, 0);
         for (int i = 0; i < wc->concurrent && started < quota; i++) { start(ws[i]); active[i] = 1; }
-        std::vector<uint64_t> tk(wc->concurrent);
-        float layer[30], cub[105], out[5];
+        // one submission and one wait per round for all walks of this thread
+        const int W = wc->concurrent;
+        std::vector<int32_t> kk(W), nn(W), aa(W);
+        std::vector<float> LL((size_t)W * 30), CC((size_t)W * 105), outs((size_t)W * 5);
+        std::vector<uint64_t> tk(W);
+        std::vector<int> idx;
         for (;;) {
-            int nact = 0;
-            for (int i = 0; i < wc->concurrent; i++) {
+            idx.clear();
+            for (int i = 0; i < W; i++) {
                 if (!active[i]) continue;
                 Walk& w = ws[i];
                 int near = -1;
-                const double dc = dist_all(w.p, &near);
-                const double dw = win_dist(w.p);
+                double dc = dist_all(w.p, &near);
+                double dw = win_dist(w.p);
                 if (!w.first && (dc < wc->stop_dist || dw < wc->stop_dist || w.hops > 100000)) {
                     if (dc < wc->stop_dist && near >= 0) {
                         lsum[near] += w.w;
@@ -196,26 +200,47 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
                     }
                     lhops += w.hops;
                     ldone++;
-                    if (started < quota) start(w); else { active[i] = 0; continue; }
+                    if (started >= quota) { active[i] = 0; continue; }
+                    // the slot starts a new walk: its first cube is sized at the new point
+                    start(w);
+                    dc = dist_all(w.p, &near);
+                    dw = win_dist(w.p);
                 }
                 double h = std::min(std::min(dc, dw), wc->max_cube);
                 if (w.first) h = std::min(h, margin);
+                if (!(h > 0)) {  // a start point on a conductor or the window: geometry error
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_bad.empty()) {
+                        char msg[160];
+                        std::snprintf(msg, sizeof msg, "walk at (%.6g, %.6g, %.6g) has no room for a cube", w.p[0],
+                                      w.p[1], w.p[2]);
+                        first_bad = msg;
+                    }
+                    failed++;
+                    active[i] = 0;
+                    continue;
+                }
                 w.h = h;
-                describe(g, w.p, h, layer, cub);
-                const int kind = w.first ? DRW_GRADIENT : DRW_POISSON;
-                if (drw_server_submit(sv, kind, N, w.axis, layer, cub, 0, &tk[i])) { failed++; active[i] = 0; continue; }
-                nact++;
+                const int j = (int)idx.size();
+                describe(g, w.p, h, &LL[(size_t)j * 30], &CC[(size_t)j * 105]);
+                kk[j] = w.first ? DRW_GRADIENT : DRW_POISSON;
+                nn[j] = N;
+                aa[j] = w.axis;
+                idx.push_back(i);
             }
-            if (nact == 0) {
-                bool any = false;
-                for (char a : active) any |= a != 0;
-                if (!any) break;
-                continue;
+            const int n = (int)idx.size();
+            if (n == 0) break;
+            if (drw_server_submit_many(sv, n, kk.data(), nn.data(), aa.data(), LL.data(), CC.data(), nullptr,
+                                       tk.data()) ||
+                drw_server_wait_many(sv, n, tk.data(), outs.data(), -1)) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_bad.empty()) first_bad = drw::last_error();
+                failed += n;
+                break;
             }
-            for (int i = 0; i < wc->concurrent; i++) {
-                if (!active[i]) continue;
-                Walk& w = ws[i];
-                if (drw_server_wait(sv, tk[i], out, -1)) { failed++; active[i] = 0; continue; }
+            for (int j = 0; j < n; j++) {
+                Walk& w = ws[idx[j]];
+                const float* out = &outs[(size_t)j * 5];
                 const int nout = w.first ? 5 : 3;
                 bool finite = true;
                 for (int k = 0; k < nout; k++) finite = finite && std::isfinite(out[k]);
@@ -234,7 +259,7 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
                         }
                     }
                     failed++;
-                    active[i] = 0;
+                    active[idx[j]] = 0;
                     continue;
                 }
                 if (w.first) {

@@ -62,6 +62,31 @@ class Server:
         check(_lib.lib().drw_server_wait(self._h, ticket, out.ctypes.data, timeout_ms), "drw_server_wait")
         return out
 
+    def submit_many(self, kinds, Ns, axes, layers, cuboids, qids=None) -> np.ndarray:
+        """Submit n queries in one call; returns their tickets (uint64 [n])."""
+        K = np.ascontiguousarray(kinds, np.int32)
+        n = K.size
+        Nn = np.ascontiguousarray(Ns, np.int32)
+        A = np.ascontiguousarray(axes, np.int32)
+        L = np.ascontiguousarray(layers, np.float32).reshape(n, 30)
+        C = np.ascontiguousarray(cuboids, np.float32).reshape(n, 105)
+        assert Nn.size == n and A.size == n
+        Q = None if qids is None else np.ascontiguousarray(qids, np.uint64)
+        t = np.zeros(n, np.uint64)
+        check(_lib.lib().drw_server_submit_many(self._h, n, K.ctypes.data, Nn.ctypes.data, A.ctypes.data,
+                                                L.ctypes.data, C.ctypes.data, None if Q is None else Q.ctypes.data,
+                                                t.ctypes.data), "drw_server_submit_many")
+        return t
+
+    def wait_many(self, tickets, timeout_ms=-1) -> np.ndarray:
+        """Wait for tickets in order; returns float32 [n, 5] (Poisson rows use
+        the first 3 columns)."""
+        T = np.ascontiguousarray(tickets, np.uint64)
+        out = np.zeros((T.size, 5), np.float32)
+        check(_lib.lib().drw_server_wait_many(self._h, T.size, T.ctypes.data, out.ctypes.data, timeout_ms),
+              "drw_server_wait_many")
+        return out
+
     def stats(self, reset=False) -> dict:
         s = ServerStats()
         check(_lib.lib().drw_server_stats_get(self._h, ctypes.byref(s), int(reset)), "drw_server_stats_get")

@@ -145,10 +145,12 @@ def _kl(pred_logp, target):
     return (target * (t.log() - pred_logp)).sum(-1).mean()
 
 
-def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log=print, seed=0, minutes=None):
+def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log=print, seed=0, minutes=None,
+              eval_every=500, n_eval=512):
     """Train one network.  With `minutes`, the run is time-boxed: the number
     of steps is set from the measured step time so that the cosine schedule
-    ends when the budget does."""
+    ends when the budget does.  Every `eval_every` steps the held-out error
+    (last 5% of the cubes) is measured and the best state is kept."""
     torch.manual_seed(seed)
     n = cubes.shape[0]
     ntr = int(n * 0.95)
@@ -162,6 +164,7 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
     faces = {"PoissonPredictor": [0, 1, 2, 3, 4, 5], "Gradient1Predictor": [0, 1],
              "Gradient2Predictor": [2, 3, 4, 5]}.get(name, [None])
     opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-10)
+    ev_idx = np.arange(ntr, min(n, ntr + n_eval))
 
     def make_batch(idx, f):
         x = Xall[idx].float()
@@ -191,9 +194,30 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
         if name == "PoissonPredictor":
             p = F.relu(y) + 1e-10
             p = p / p.sum(-1, keepdim=True)
-            return _kl(p.log(), tgt) + 0.2 * (p - tgt).abs().sum(-1).mean()
+            # the Poisson kernel is positive on every tile: a negative raw output
+            # is always wrong and gets no gradient through the relu head, so it
+            # is pushed back up directly (keeps the head from dying)
+            revive = F.relu(-y).mean() / (y.abs().mean().detach() + 1e-6)
+            return _kl(p.log(), tgt) + 0.2 * (p - tgt).abs().sum(-1).mean() + 0.1 * revive
         p = y / y.abs().sum(-1, keepdim=True)
         return F.mse_loss(p, tgt) * 529.0 * 50.0 + 0.2 * (p - tgt).abs().sum(-1).mean()
+
+    def held_out():
+        model.eval()
+        if name.endswith("Predictor"):
+            e = evaluate_face_l1(name, model, cubes, G, Gz, ev_idx, dev=dev)
+        else:
+            with torch.no_grad():
+                x = Xall[ev_idx].float()
+                out = model(x)
+                if name == "PoissonSelector":
+                    t = Gp[ev_idx].sum(-1)
+                    e = (F.softmax(out, -1) - t / t.sum(-1, keepdim=True)).abs().sum(-1).mean().item()
+                else:
+                    tot = Gg[ev_idx].abs().sum(-1)
+                    e = (F.softmax(out[:, :6], -1) - tot / tot.sum(-1, keepdim=True)).abs().sum(-1).mean().item()
+        model.train()
+        return e
 
     def step(st, total):
         f = faces[st % len(faces)]
@@ -225,18 +249,26 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=2e-3, dev="cuda", log
         if dev != "cpu":
             torch.cuda.synchronize()
         per = (time.time() - t1) / 30
-        total = max(100, int((minutes * 60 - (time.time() - t0)) / per))
+        total = max(100, int((minutes * 60 - (time.time() - t0)) / (per * (1 + 2.0 / max(eval_every, 1)))))
+        log(f"{name}: {per * 1e3:.1f} ms/step at batch {batch} -> {total} steps")
     else:
         total = epochs * max(1, (ntr * len(faces)) // batch)
+    best, best_state = float("inf"), None
     tot_l, nb = 0.0, 0
     for st in range(total):
         loss = step(st, total)
         if st % 50 == 0:
             tot_l += loss.item()
             nb += 1
-        if st % max(1, total // 10) == 0 or st == total - 1:
-            log(f"{name} step {st + 1}/{total} loss {tot_l / max(nb, 1):.5f} ({time.time() - t0:.0f}s)")
+        if (st + 1) % eval_every == 0 or st == total - 1:
+            e = held_out()
+            if e < best:
+                best, best_state = e, {k: v.detach().clone() for k, v in model.state_dict().items()}
+            log(f"{name} step {st + 1}/{total} loss {tot_l / max(nb, 1):.5f} held-out L1 {e:.4f} "
+                f"(best {best:.4f}, {time.time() - t0:.0f}s)")
             tot_l, nb = 0.0, 0
+    if best_state is not None:
+        model.load_state_dict(best_state)
     model.eval()
     return model
 
@@ -264,41 +296,96 @@ def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda"):
     return float(np.mean(errs))
 
 
-def train_all(out_dir, n_cubes=40000, epochs=30, seed=0, log=print, dev="cuda", minutes=None, batch=256):
+def save_dataset(path, cubes, G, Gz):
+    np.save(path + ".cubes.npy", cubes)
+    np.save(path + ".G.npy", G)
+    np.save(path + ".Gz.npy", Gz)
+
+
+def load_dataset(path):
+    return (np.load(path + ".cubes.npy", mmap_mode="r"), np.load(path + ".G.npy", mmap_mode="r"),
+            np.load(path + ".Gz.npy", mmap_mode="r"))
+
+
+SHARE = {"PoissonSelector": 0.12, "PoissonPredictor": 0.38, "GradientSelectorWeight": 0.10,
+         "Gradient2Predictor": 0.24, "Gradient1Predictor": 0.16}
+
+
+def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log):
+    m = train_net(name, np.ascontiguousarray(cubes), np.ascontiguousarray(G), np.ascontiguousarray(Gz), dev="cuda",
+                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes)
+    os.makedirs(out_dir, exist_ok=True)
+    W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), export_state(m))
+    return m
+
+
+def train_all(out_dir, n_cubes=40000, epochs=30, seed=0, log=print, dev="cuda", minutes=None, batch=256, lr=1e-3):
+    """Sequential training of the five networks (`minutes` split by SHARE)."""
     t0 = time.time()
     cubes, G, Gz = fdm_dataset(n_cubes, seed, dev=dev)
     log(f"dataset: {n_cubes} cubes, FDM kernels in {time.time() - t0:.1f}s")
-    states, report = {}, {}
-    ho = np.arange(int(n_cubes * 0.95), n_cubes)
-    share = {"PoissonSelector": 0.08, "PoissonPredictor": 0.32, "GradientSelectorWeight": 0.08,
-             "Gradient2Predictor": 0.32, "Gradient1Predictor": 0.20}
+    states = {}
     for name in W.MODEL_ORDER:
-        m = train_net(name, cubes, G, Gz, epochs=epochs, dev=dev, log=log, batch=batch,
-                      minutes=None if minutes is None else minutes * share[name])
+        m = train_net(name, cubes, G, Gz, epochs=epochs, dev=dev, log=log, batch=batch, lr=lr,
+                      minutes=None if minutes is None else minutes * SHARE[name])
         states[name] = export_state(m)
-        if name.endswith("Predictor"):
-            report[name] = evaluate_face_l1(name, m, cubes, G, Gz, ho, dev=dev)
-            log(f"{name}: held-out per-face L1 {report[name]:.4f}")
         os.makedirs(out_dir, exist_ok=True)
         W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), states[name])  # checkpoint per network
     path = W.export_model_dir(out_dir, states)
     log(f"exported {path} in {time.time() - t0:.0f}s")
-    return path, report
+    return path
+
+
+def train_parallel(out_dir, data, minutes, batch, lr, seed, log=print):
+    """One process per network on the same GPU, each for the whole budget:
+    the networks are small, so a single training step leaves the GPU mostly
+    idle and five concurrent processes train about five times as much."""
+    import subprocess
+    import sys
+    procs = []
+    for i, name in enumerate(W.MODEL_ORDER):
+        cmd = [sys.executable, "-u", "-m", "deeprwcap_amd.training", "--out", out_dir, "--data", data, "--only", name,
+               "--minutes", str(minutes), "--batch", str(batch), "--lr", str(lr), "--seed", str(seed + i)]
+        procs.append((name, subprocess.Popen(cmd)))
+    rc = 0
+    for name, p in procs:
+        r = p.wait()
+        log(f"{name}: exit {r}")
+        rc = rc or r
+    if rc:
+        raise RuntimeError("a training process failed")
+    states = {n: W.read_drwnet(os.path.join(out_dir, f"{n}.drwnet")) for n in W.MODEL_ORDER}
+    return W.export_model_dir(out_dir, states)
 
 
 if __name__ == "__main__":
     import argparse
     ap = argparse.ArgumentParser(description="FDM dataset + training + .drwnet export")
-    ap.add_argument("--out", required=True)
-    ap.add_argument("--n", type=int, default=40000)
+    ap.add_argument("--out", help="model directory (models.txt + .drwnet files)")
+    ap.add_argument("--n", type=int, default=40000, help="cubes in the FDM dataset")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--minutes", type=float, default=None, help="total wall-clock budget for the five nets")
+    ap.add_argument("--minutes", type=float, default=None,
+                    help="wall-clock budget (sequential: for all five nets; --parallel/--only: per net)")
     ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--make-data", metavar="PATH", help="only build the FDM dataset and save it under PATH.*.npy")
+    ap.add_argument("--data", metavar="PATH", help="use a dataset saved by --make-data")
+    ap.add_argument("--only", help="train this network only (needs --data)")
+    ap.add_argument("--parallel", action="store_true", help="one process per network (needs --data)")
     args = ap.parse_args()
 
     def _log(msg):
         print(msg, flush=True)
 
-    train_all(args.out, n_cubes=args.n, epochs=args.epochs, seed=args.seed, log=_log, minutes=args.minutes,
-              batch=args.batch)
+    if args.make_data:
+        t0 = time.time()
+        save_dataset(args.make_data, *fdm_dataset(args.n, args.seed))
+        _log(f"dataset: {args.n} cubes in {time.time() - t0:.1f}s -> {args.make_data}.*.npy")
+    elif args.only:
+        train_one(args.out, args.only, *load_dataset(args.data), args.minutes, args.batch, args.lr, args.seed, _log)
+    elif args.parallel:
+        _log(f"exported {train_parallel(args.out, args.data, args.minutes, args.batch, args.lr, args.seed, _log)}")
+    else:
+        train_all(args.out, n_cubes=args.n, epochs=args.epochs, seed=args.seed, log=_log, minutes=args.minutes,
+                  batch=args.batch, lr=args.lr)

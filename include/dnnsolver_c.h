@@ -35,6 +35,7 @@ extern "C" {
 #define DRW_EHIP (-3)     /* HIP runtime error */
 #define DRW_ESTATE (-4)   /* object not ready (e.g. models not loaded) */
 #define DRW_EIO (-5)      /* file missing or malformed */
+#define DRW_ETIMEOUT (-6) /* a wait ran out of time; the ticket stays valid */
 
 enum drw_kind { DRW_POISSON = 0, DRW_GRADIENT = 1 };
 enum drw_backend { DRW_BACKEND_NN = 0, DRW_BACKEND_FDM = 1 };
@@ -188,9 +189,21 @@ int drw_server_create(int device, const char* models_txt, const drw_server_confi
  * layer[30], cuboid[105]; qid = Philox counter of this query (0: the ticket). */
 int drw_server_submit(drw_server* sv, int kind, int N, int axis, const float* layer, const float* cuboid, uint64_t qid,
                       uint64_t* ticket);
-/* Blocks until the query is done (timeout_ms < 0: no limit); out receives 3
- * or 5 floats.  Each ticket can be waited for once. */
+/* Blocks until the query is done (timeout_ms < 0: no limit; DRW_ETIMEOUT when
+ * the limit passes); out receives 3 or 5 floats.  Each ticket can be waited
+ * for once; at most 65536 tickets can be outstanding (submitted and not yet
+ * waited for). */
 int drw_server_wait(drw_server* sv, uint64_t ticket, float* out, int timeout_ms);
+/* Batch forms for callers with many queries in flight (a walker thread with
+ * many walks): one call submits n queries (kinds/Ns/axes[n], layers[n][30],
+ * cuboids[n][105], qids[n] or NULL) and returns their tickets; one call waits
+ * for n tickets in order, outs[n][5] (Poisson results use the first 3).  On
+ * DRW_ETIMEOUT the tickets before the one that timed out are consumed and the
+ * rest stay valid.  No counterpart in the reference, whose walker submits one
+ * task at a time (parallel_sampler.cpp:42-70). */
+int drw_server_submit_many(drw_server* sv, int n, const int32_t* kinds, const int32_t* Ns, const int32_t* axes,
+                           const float* layers, const float* cuboids, const uint64_t* qids, uint64_t* tickets);
+int drw_server_wait_many(drw_server* sv, int n, const uint64_t* tickets, float* outs, int timeout_ms);
 int drw_server_stats_get(drw_server* sv, drw_server_stats* stats, int reset);
 /* Drains pending queries, then frees everything. */
 void drw_server_destroy(drw_server* sv);

@@ -73,3 +73,19 @@ def test_server_matches_solver(models_txt):
     got = [sv.wait(sv.submit(0, 23, 2, L[b], C[b], int(qid[b])), 0) for b in range(B)]
     sv.close()
     np.testing.assert_array_equal(np.stack(got), ref)
+
+
+def test_submit_many_matches_single(models_txt):
+    """The batch submit/wait calls return exactly what one-at-a-time calls
+    return for the same query ids."""
+    from deeprwcap_amd.serving import Server
+    reqs = _requests(96, 5)
+    sv = Server(0, models_txt, max_batch=32, max_inflight=3)
+    single = [sv.wait(sv.submit(k, n, a, L, C, q), k) for k, n, a, L, C, q in reqs]
+    kinds = [r[0] for r in reqs]
+    t = sv.submit_many(kinds, [r[1] for r in reqs], [r[2] for r in reqs], np.stack([r[3] for r in reqs]),
+                       np.stack([r[4] for r in reqs]), np.array([r[5] for r in reqs], np.uint64))
+    many = sv.wait_many(t)
+    sv.close()
+    for i, (k, s) in enumerate(zip(kinds, single)):
+        np.testing.assert_array_equal(many[i, :len(s)], s, err_msg=f"request {i} {reqs[i][:3]}")
