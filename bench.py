@@ -119,6 +119,19 @@ def cpu_baseline(L, C, models_txt, seconds=15.0):
                       f"PoissonPredictor, tile draw, locate) in FP32 PyTorch on {cores} threads, {dt:.1f}s"}
 
 
+def mixed_walkers(models_txt, n=1 << 20):
+    """1,048,576 walkers' queries in one pass: N uniform over {16, 21, 23},
+    half Poisson / half gradient, networks at N = 23, FDM at 16 and 21."""
+    from deeprwcap_amd.mixed import MixedRunner, make_queries
+    q = make_queries(n, seed=5)
+    runner = MixedRunner(models_txt)
+    runner.run(q, subset=np.arange(4096))  # warm-up
+    out, groups, total = runner.run(q)
+    log(f"mixed walkers: {n} queries in {total:.3f} s ({n / total:,.0f} queries/s)")
+    return {"queries": n, "seconds": total, "queries_per_s": n / total, "groups": groups,
+            "mix": "N uniform {16,21,23}; Poisson/gradient 50/50; N=23 networks, N=16/21 FDM (tol 1e-6)"}
+
+
 def serving_runs(models_txt, rates, duration_s=2.0, threads=64):
     """Serving loop: 64 host threads issue mixed Poisson/gradient N = 23
     queries with Poisson arrivals; returns per-rate latency and throughput."""
@@ -154,6 +167,7 @@ def main():
     ap.add_argument("--fdm-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -257,6 +271,8 @@ def main():
                      "traffic": None,
                      "note": "algorithmic 1,292,876 FLOP per query; most of it is depthwise conv + GELU on the VALU"},
     }
+    if not args.no_mixed:
+        res["mixed_walkers"] = mixed_walkers(models_txt)
     if not args.no_serving:
         res["serving"] = serving_runs(models_txt, [1e5, 1e6, 1e7, 0])
         res["p99_latency_ms"] = res["serving"][1]["p99_ms"]

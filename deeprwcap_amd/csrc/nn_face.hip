@@ -27,7 +27,7 @@ namespace drw {
 // registers in either role), so a lane ends with 4 consecutive channels of
 // one pixel and writes them with one 8-byte store.
 template <int CIN_PAD, int COUT, int COUT_PAD, int CSI, int CSO>
-__device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const float* bias)
+__device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const float* bias, const float2* gtab)
 {
     constexpr int KT = CIN_PAD / 16, NT = COUT_PAD / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -59,7 +59,7 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
             if (V == VFULL) {
                 half4_t o;
 #pragma unroll
-                for (int i = 0; i < 4; i++) o[i] = (half_t)gelu(acc[i] + bb[n][i]);
+                for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i] + bb[n][i], gtab);
                 *(half4_t*)(orow + n * 16 + kq) = o;
             } else {
                 static_assert(COUT % 16 == 0 || COUT % 16 == 8, "tiles hold 16 or 8 valid channels");
@@ -70,8 +70,8 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
                 const int c0 = (hi ? 2 : 0) + 4 * ((lane >> 4) & 1);
                 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
                 h2_t o;
-                o[0] = (half_t)gelu((hi ? t2 : acc[0]) + bb[n][0]);
-                o[1] = (half_t)gelu((hi ? t3 : acc[1]) + bb[n][1]);
+                o[0] = (half_t)gelu_tab((hi ? t2 : acc[0]) + bb[n][0], gtab);
+                o[1] = (half_t)gelu_tab((hi ? t3 : acc[1]) + bb[n][1], gtab);
                 *(h2_t*)(orow + n * 16 + c0) = o;
             }
         }
@@ -104,15 +104,26 @@ __device__ __forceinline__ void store_pair(half_t* d, float a, float b)
 }
 __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float2*)d = make_float2(a, b); }
 
+// GELU of a value stored as TO: the LDS table where the result is rounded to
+// fp16 anyway, the exact-form approximation for fp32 outputs.
+template <typename TO>
+__device__ __forceinline__ float gelu_for(float x, const float2* gtab)
+{
+    if constexpr (sizeof(TO) == 2) return gelu_tab(x, gtab);
+    else return gelu(x);
+}
+
 // Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU, one
 // output (pixel, channel pair) per item with all 9 taps read from LDS: every
-// item is independent, so all four waves stay busy and the loads of many
-// items are in flight at once.  Channel pairs run fastest within a wave, so
-// each tap is one contiguous 64-lane read.  Needs 256 % (C/2) == 0 (a
-// thread's channel pair, and so its 18 weights, stays fixed); other channel
-// counts use the sliding form below.
+// item is independent, so all four waves stay busy.  The taps are
+// branch-free: a tap outside the image reads `zero` (a zeroed LDS slot)
+// instead, so the nine loads of an item issue back to back.  Channel pairs
+// run fastest within a wave, so each tap is one contiguous 64-lane read.
+// Needs 256 % (C/2) == 0 (a thread's channel pair, and so its 18 weights,
+// stays fixed); other channel counts use the sliding form below.
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
-__device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, const float* w, const float* bias)
+__device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, const float* w, const float* bias,
+                               const TI* zero, const float2* gtab)
 {
     constexpr int CP = C / 2;
     static_assert(256 % CP == 0, "channel pair fixed per thread");
@@ -125,22 +136,25 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
     for (int item = threadIdx.x; item < CP * kNN; item += 256) {
         const int p = item / CP;
         const int y = p / kN, x = p - y * kN;
-        float a0 = b0, a1 = b1;
+        const TI* c = in + p * CSI + 2 * cp;
+        const bool rv[3] = {y >= DIL, true, y + DIL < kN};
+        const bool cv[3] = {x >= DIL, true, x + DIL < kN};
+        float v0[9], v1[9];
 #pragma unroll
         for (int ky = 0; ky < 3; ky++) {
-            const int yy = y + (ky - 1) * DIL;
-            if (yy < 0 || yy >= kN) continue;
 #pragma unroll
             for (int kx = 0; kx < 3; kx++) {
-                const int xx = x + (kx - 1) * DIL;
-                if (xx < 0 || xx >= kN) continue;
-                float v0, v1;
-                load_pair(in + (yy * kN + xx) * CSI + 2 * cp, v0, v1);
-                a0 += w0[3 * ky + kx] * v0;
-                a1 += w1[3 * ky + kx] * v1;
+                const TI* s = (rv[ky] && cv[kx]) ? c + ((ky - 1) * DIL * kN + (kx - 1) * DIL) * CSI : zero;
+                load_pair(s, v0[3 * ky + kx], v1[3 * ky + kx]);
             }
         }
-        store_pair(out + p * CSO + 2 * cp, gelu(a0), gelu(a1));
+        float a0 = b0, a1 = b1;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            a0 += w0[k] * v0[k];
+            a1 += w1[k] * v1[k];
+        }
+        store_pair(out + p * CSO + 2 * cp, gelu_for<TO>(a0, gtab), gelu_for<TO>(a1, gtab));
     }
 }
 
@@ -151,7 +165,7 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
 // are reused from the previous step) and the column's x-border mask is
 // computed once.
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
-__device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float* bias)
+__device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float* bias, const float2* gtab)
 {
     constexpr int CP = C / 2;
     for (int item = threadIdx.x; item < CP * kN; item += 256) {
@@ -185,7 +199,7 @@ __device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float
                     a0 += w0[kx] * up[2 * kx] + w0[3 + kx] * mid[2 * kx] + w0[6 + kx] * dn[2 * kx];
                     a1 += w1[kx] * up[2 * kx + 1] + w1[3 + kx] * mid[2 * kx + 1] + w1[6 + kx] * dn[2 * kx + 1];
                 }
-                store_pair(out + (y * kN + x) * CSO + 2 * cp, gelu(a0), gelu(a1));
+                store_pair(out + (y * kN + x) * CSO + 2 * cp, gelu_for<TO>(a0, gtab), gelu_for<TO>(a1, gtab));
 #pragma unroll
                 for (int q = 0; q < 6; q++) { up[q] = mid[q]; mid[q] = dn[q]; }
             }
@@ -196,10 +210,11 @@ __device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float
 // Depthwise layer: direct form when the channel pair can stay fixed per
 // thread, sliding form otherwise (24 channels).
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
-__device__ __forceinline__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias)
+__device__ __forceinline__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias, const float* zero,
+                                        const float2* gtab)
 {
-    if constexpr (256 % (C / 2) == 0) dw_conv_direct<C, DIL, CSI, CSO>(in, out, w, bias);
-    else dw_conv_slide<C, DIL, CSI, CSO>(in, out, w, bias);
+    if constexpr (256 % (C / 2) == 0) dw_conv_direct<C, DIL, CSI, CSO>(in, out, w, bias, (const TI*)zero, gtab);
+    else dw_conv_slide<C, DIL, CSI, CSO>(in, out, w, bias, gtab);
 }
 
 // Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4), fp32
@@ -245,7 +260,7 @@ __device__ void head_conv(const float* in, float* hout, const float* w, float b)
 // masks of the two fixed axes are AND-ed once per pixel.
 template <int MODE, int COUT, int CSO>
 __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out, const half4_t* frag,
-                              const float* bias)
+                              const float* bias, const float2* gtab)
 {
     static_assert(COUT % 16 == 0, "full MFMA column tiles only");
     constexpr int KT = 2, NT = COUT / 16;
@@ -314,7 +329,7 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
             for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
             half4_t o;
 #pragma unroll
-            for (int i = 0; i < 4; i++) o[i] = (half_t)gelu(acc[i] + bb[n][i]);
+            for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i] + bb[n][i], gtab);
             *(half4_t*)(out + p * CSO + n * 16 + kq) = o;
         }
     }
@@ -330,6 +345,8 @@ template <int CS>
 struct FaceSmem {
     alignas(16) half_t X[kPix * CS];  // 16-byte aligned: read as half4 / float2
     alignas(16) half_t Y[kPix * CS];
+    alignas(16) float zero[4];  // read by out-of-image depthwise taps; zeroed at kernel start
+    alignas(16) float2 gtab[kGeluTab];  // GELU table of the fp16-output layers, built at kernel start
     VoxDesc desc;
     float red[8];
     int ibuf[4];
@@ -339,39 +356,39 @@ struct FaceSmem {
 template <int MODE>
 __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0]); __syncthreads();
-    dw_conv<16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0]); __syncthreads();
-    pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.pw_bias[1]); __syncthreads();
-    dw_conv<16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1]); __syncthreads();
-    pw_mfma<16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.pw_bias[2]); __syncthreads();
+    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0], s.gtab); __syncthreads();
+    dw_conv<16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
+    pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.pw_bias[1], s.gtab); __syncthreads();
+    dw_conv<16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
+    pw_mfma<16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.pw_bias[2], s.gtab); __syncthreads();
     // fp32 tail
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
-    dw_conv<8, 2, 16, 8>(s.X, fY, w.dw_w[2], w.dw_b[2]); __syncthreads();
+    dw_conv<8, 2, 16, 8>(s.X, fY, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
     pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[3], w.pw_bias[3]); __syncthreads();
-    dw_conv<4, 3, 4, 4>(fX, fY, w.dw_w[3], w.dw_b[3]); __syncthreads();
+    dw_conv<4, 3, 4, 4>(fX, fY, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
     pw_valu<4, 2, 4, 2>(fY, fX, w.pw_w[4], w.pw_bias[4]); __syncthreads();
     head_conv<2, 2>(fX, fY, w.head_w, w.head_b);
 }
 
 __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0]); __syncthreads();
-    dw_conv<32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0]); __syncthreads();
-    pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.pw_bias[1]); __syncthreads();
-    dw_conv<32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1]); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.pw_bias[2]); __syncthreads();
-    dw_conv<24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2]); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.pw_bias[3]); __syncthreads();
-    dw_conv<24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3]); __syncthreads();
-    pw_mfma<32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.pw_bias[4]); __syncthreads();
-    dw_conv<16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4]); __syncthreads();
-    pw_mfma<16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.pw_bias[5]); __syncthreads();
-    dw_conv<16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5]); __syncthreads();
-    pw_mfma<16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.pw_bias[6]); __syncthreads();
+    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0], s.gtab); __syncthreads();
+    dw_conv<32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
+    pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.pw_bias[1], s.gtab); __syncthreads();
+    dw_conv<32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
+    pw_mfma<32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.pw_bias[2], s.gtab); __syncthreads();
+    dw_conv<24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
+    pw_mfma<32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.pw_bias[3], s.gtab); __syncthreads();
+    dw_conv<24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
+    pw_mfma<32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.pw_bias[4], s.gtab); __syncthreads();
+    dw_conv<16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
+    pw_mfma<16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.pw_bias[5], s.gtab); __syncthreads();
+    dw_conv<16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
+    pw_mfma<16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.pw_bias[6], s.gtab); __syncthreads();
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
-    dw_conv<8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6]); __syncthreads();
+    dw_conv<8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
     pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7]); __syncthreads();
     head_conv<4, 4>(fX, fY, w.head_w, w.head_b);
 }
@@ -401,6 +418,8 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     if (MODE == 1 && !(face == 0 || face == 1)) return;
     if (MODE == 2 && !(face >= 2 && face <= 5)) return;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
+    if (threadIdx.x < 4) s.zero[threadIdx.x] = 0.f;
+    gelu_table_build(s.gtab);
     __syncthreads();
     if constexpr (MODE == 2) face_solver_plus(s, w, face, axis); else face_solver<MODE>(s, w, face, axis);
     __syncthreads();

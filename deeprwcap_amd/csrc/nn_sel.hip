@@ -126,10 +126,34 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
     __syncthreads();
     const int axis = a.axis ? a.axis[b] : 2;
-    for (int v = threadIdx.x; v < kNNN; v += 256) {
-        int z = v / kNN, y = (v / kN) % kN, x = v % kN;
-        if (a.axis) remap_axis(axis, z, y, x);
-        s.cube[v] = (half_t)vox_value(s.desc, z, y, x);
+    // Cube in the (axis pre-rotated) frame, built row by row: along an
+    // x-row of the output exactly one source axis varies (+-1 per voxel), so
+    // the cuboid masks of the two fixed source axes are AND-ed once per row
+    // chunk; items are quarter rows (6, 6, 6, 5 voxels) to balance the waves.
+    for (int it = threadIdx.x; it < kNN * 4; it += 256) {
+        const int r = it >> 2, q = it & 3;  // row r = 23 z + y of the output frame
+        const int zo = r / kN, yo = r - zo * kN;
+        int z0 = zo, y0 = yo, x0 = 0, z1 = zo, y1 = yo, x1 = 1;
+        if (a.axis) {
+            remap_axis(axis, z0, y0, x0);
+            remap_axis(axis, z1, y1, x1);
+        }
+        const VoxDesc& d = s.desc;
+        const uint16_t* vm;
+        int base, dir;
+        uint32_t fixedm;
+        const bool zvar = z1 != z0;
+        if (zvar) { vm = d.bz; base = z0; dir = z1 - z0; fixedm = (uint32_t)d.by[y0] & d.bx[x0]; }
+        else if (y1 != y0) { vm = d.by; base = y0; dir = y1 - y0; fixedm = (uint32_t)d.bz[z0] & d.bx[x0]; }
+        else { vm = d.bx; base = x0; dir = x1 - x0; fixedm = (uint32_t)d.bz[z0] & d.by[y0]; }
+        const float lfix = d.lz[z0];
+        const int xa = 6 * q, xb = q == 3 ? kN : xa + 6;
+        half_t* dst = s.cube + r * kN;
+        for (int x = xa; x < xb; x++) {
+            const int k = base + dir * x;
+            const uint32_t m = fixedm & vm[k];
+            dst[x] = (half_t)(m ? d.cv[__builtin_ctz(m)] : (zvar ? d.lz[k] : lfix));
+        }
     }
     __syncthreads();
     conv3d_mfma<1, C1, 23, 11, false>(s.cube, s.l1, w.frag[0], w.bias[0], s.koff);

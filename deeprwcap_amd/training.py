@@ -146,11 +146,16 @@ def _kl(pred_logp, target):
 
 
 def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log=print, seed=0, minutes=None,
-              eval_every=500, n_eval=512):
+              eval_every=500, n_eval=512, loss_kind="kl", ema=0.999):
     """Train one network.  With `minutes`, the run is time-boxed: the number
     of steps is set from the measured step time so that the cosine schedule
-    ends when the budget does.  Every `eval_every` steps the held-out error
-    (last 5% of the cubes) is measured and the best state is kept."""
+    ends when the budget does.  An exponential moving average of the weights
+    (decay `ema`, 0 = off) is what gets evaluated and exported.  Every
+    `eval_every` steps the held-out error (last 5% of the cubes) is measured
+    and the best state is kept.  `loss_kind` for the PoissonPredictor: "kl"
+    (KL + 0.2 L1, the reference's main loss plus an L1 term) or "l1" (L1 +
+    MSE of the normalised distribution: bounded gradients, and L1 is the
+    quantity the accuracy gate measures)."""
     torch.manual_seed(seed)
     n = cubes.shape[0]
     ntr = int(n * 0.95)
@@ -165,6 +170,8 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
              "Gradient2Predictor": [2, 3, 4, 5]}.get(name, [None])
     opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-10)
     ev_idx = np.arange(ntr, min(n, ntr + n_eval))
+    avg = torch.optim.swa_utils.AveragedModel(
+        model, multi_avg_fn=torch.optim.swa_utils.get_ema_multi_avg_fn(ema), use_buffers=True) if ema > 0 else None
 
     def make_batch(idx, f):
         x = Xall[idx].float()
@@ -198,25 +205,28 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
             # is always wrong and gets no gradient through the relu head, so it
             # is pushed back up directly (keeps the head from dying)
             revive = F.relu(-y).mean() / (y.abs().mean().detach() + 1e-6)
+            if loss_kind == "l1":
+                return (p - tgt).abs().sum(-1).mean() + 529.0 * F.mse_loss(p, tgt) + 0.1 * revive
             return _kl(p.log(), tgt) + 0.2 * (p - tgt).abs().sum(-1).mean() + 0.1 * revive
         p = y / y.abs().sum(-1, keepdim=True)
         return F.mse_loss(p, tgt) * 529.0 * 50.0 + 0.2 * (p - tgt).abs().sum(-1).mean()
 
     def held_out():
-        model.eval()
+        m = avg.module if avg is not None else model
+        m.eval()
         if name.endswith("Predictor"):
-            e = evaluate_face_l1(name, model, cubes, G, Gz, ev_idx, dev=dev)
+            e = evaluate_face_l1(name, m, cubes, G, Gz, ev_idx, dev=dev)
         else:
             with torch.no_grad():
                 x = Xall[ev_idx].float()
-                out = model(x)
+                out = m(x)
                 if name == "PoissonSelector":
                     t = Gp[ev_idx].sum(-1)
                     e = (F.softmax(out, -1) - t / t.sum(-1, keepdim=True)).abs().sum(-1).mean().item()
                 else:
                     tot = Gg[ev_idx].abs().sum(-1)
                     e = (F.softmax(out[:, :6], -1) - tot / tot.sum(-1, keepdim=True)).abs().sum(-1).mean().item()
-        model.train()
+        m.train()
         return e
 
     def step(st, total):
@@ -232,6 +242,8 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         for gr in opt.param_groups:
             gr["lr"] = lr * max(frac, 0.02)
         opt.step()
+        if avg is not None:
+            avg.update_parameters(model)
         return loss
 
     model.train()
@@ -263,7 +275,8 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         if (st + 1) % eval_every == 0 or st == total - 1:
             e = held_out()
             if e < best:
-                best, best_state = e, {k: v.detach().clone() for k, v in model.state_dict().items()}
+                src = avg.module if avg is not None else model
+                best, best_state = e, {k: v.detach().clone() for k, v in src.state_dict().items()}
             log(f"{name} step {st + 1}/{total} loss {tot_l / max(nb, 1):.5f} held-out L1 {e:.4f} "
                 f"(best {best:.4f}, {time.time() - t0:.0f}s)")
             tot_l, nb = 0.0, 0
@@ -311,9 +324,9 @@ SHARE = {"PoissonSelector": 0.12, "PoissonPredictor": 0.38, "GradientSelectorWei
          "Gradient2Predictor": 0.24, "Gradient1Predictor": 0.16}
 
 
-def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log):
+def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log, loss_kind="kl", ema=0.999):
     m = train_net(name, np.ascontiguousarray(cubes), np.ascontiguousarray(G), np.ascontiguousarray(Gz), dev="cuda",
-                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes)
+                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes, loss_kind=loss_kind, ema=ema)
     os.makedirs(out_dir, exist_ok=True)
     W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), export_state(m))
     return m
@@ -369,6 +382,8 @@ if __name__ == "__main__":
                     help="wall-clock budget (sequential: for all five nets; --parallel/--only: per net)")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--loss", default="kl", choices=("kl", "l1"), help="PoissonPredictor loss")
+    ap.add_argument("--ema", type=float, default=0.999, help="weight EMA decay (0: off)")
     ap.add_argument("--make-data", metavar="PATH", help="only build the FDM dataset and save it under PATH.*.npy")
     ap.add_argument("--data", metavar="PATH", help="use a dataset saved by --make-data")
     ap.add_argument("--only", help="train this network only (needs --data)")
@@ -383,7 +398,8 @@ if __name__ == "__main__":
         save_dataset(args.make_data, *fdm_dataset(args.n, args.seed))
         _log(f"dataset: {args.n} cubes in {time.time() - t0:.1f}s -> {args.make_data}.*.npy")
     elif args.only:
-        train_one(args.out, args.only, *load_dataset(args.data), args.minutes, args.batch, args.lr, args.seed, _log)
+        train_one(args.out, args.only, *load_dataset(args.data), args.minutes, args.batch, args.lr, args.seed, _log,
+                  loss_kind=args.loss, ema=args.ema)
     elif args.parallel:
         _log(f"exported {train_parallel(args.out, args.data, args.minutes, args.batch, args.lr, args.seed, _log)}")
     else:
