@@ -45,6 +45,20 @@ def models_txt_or_random(path):
     return W.export_model_dir(d, {n: W.random_state(n, i) for i, n in enumerate(W.MODEL_ORDER)}), "random"
 
 
+def measured_traffic(batch):
+    """HBM bytes per launch of face_query_kernel<0> from the committed
+    rocprofv3 counter summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+    calibration), for the batch it was measured at; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "face_query_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("batch") != batch:
+        return None
+    return 1024.0 * (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"])
+
+
 def stage_times(solver):
     import ctypes
     from deeprwcap_amd import _lib
@@ -132,6 +146,30 @@ def mixed_walkers(models_txt, n=1 << 20):
             "mix": "N uniform {16,21,23}; Poisson/gradient 50/50; N=23 networks, N=16/21 FDM (tol 1e-6)"}
 
 
+def capacitance_check(models_txt, walks):
+    """Config 5 end to end: C[0][:] of two orthogonal layers of 8 wires (0.1 x
+    0.1 x 2 um, 0.2 um pitch, eps 3.9 / 2.7 / 7.0) by the floating random walk,
+    once with network queries and once with FDM-only queries at N = 23."""
+    from deeprwcap_amd import structures as S
+    from deeprwcap_amd.serving import Server
+    geo = S.wire_crossing(n_wires=8)
+    out = {"walks": walks, "structure": "2 x 8 crossing wires", "master": 0}
+    for name, mt, tol in (("network", models_txt, 1e-6), ("fdm", None, 1e-6)):
+        sv = Server(0, mt, max_batch=4096, max_inflight=4, hbm_budget_mb=8192, fdm_tol=tol)
+        t0 = time.perf_counter()
+        cap, err, st = sv.capacitance(geo, 0, walks=walks, threads=16, concurrent=256, N=23, seed=7)
+        dt = time.perf_counter() - t0
+        sv.close()
+        out[name] = {"C": cap.tolist(), "stderr": err.tolist(), "seconds": dt, "hops_per_walk": st["hops_per_walk"]}
+        log(f"{name}: C00 {cap[0]:.6g} +- {err[0]:.2g}, {dt:.1f} s, {st['hops_per_walk']:.1f} hops/walk")
+    a, b = np.array(out["network"]["C"]), np.array(out["fdm"]["C"])
+    ea, eb = np.array(out["network"]["stderr"]), np.array(out["fdm"]["stderr"])
+    out["rel_diff_self"] = float((a[0] - b[0]) / b[0])
+    out["rel_diff_total_coupling"] = float((a[1:].sum() - b[1:].sum()) / b[1:].sum())
+    out["rel_stderr_self"] = float(np.hypot(ea[0], eb[0]) / abs(b[0]))
+    return out
+
+
 def serving_runs(models_txt, rates, duration_s=2.0, threads=64):
     """Serving loop: 64 host threads issue mixed Poisson/gradient N = 23
     queries with Poisson arrivals; returns per-rate latency and throughput."""
@@ -168,6 +206,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--capacitance", type=int, default=0, metavar="WALKS",
+                    help="only run the end-to-end check: capacitance row of the 2x8-wire structure from network "
+                         "queries vs FDM-only queries (N = 23), WALKS walks each")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -188,6 +229,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copied 3 x 1 GiB")
+        return
+    if args.capacitance:
+        models_txt, wkind = models_txt_or_random(args.models)
+        print(json.dumps(capacitance_check(models_txt, args.capacitance)), flush=True)
         return
     B = args.batch
     L, C, _ = S.layered_boxes(B, 23, seed=0)
@@ -243,6 +288,7 @@ def main():
     lat = np.array(lat[20:]) * 1e3
 
     achieved = B * PRED_FLOP / (face_ms * 1e-3) / 1e12
+    traffic = measured_traffic(B)
     res = {
         "metric": "transition queries/s at N=23 (bf16 NN vs FP32 FDM, <=1e-2 L1); p99 latency",
         "value": value,
@@ -268,8 +314,9 @@ def main():
                                 "face_predictor": face_ms},
         "roofline": {"kernel": "face_query_kernel<0> (PoissonPredictor)", "bound": "mfma", "achieved": achieved,
                      "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP16_PEAK_TFLOPS,
-                     "traffic": None,
-                     "note": "algorithmic 1,292,876 FLOP per query; most of it is depthwise conv + GELU on the VALU"},
+                     "traffic": traffic,
+                     "note": "algorithmic 1,292,876 FLOP per query; the kernel is bound by VALU issue "
+                             "(depthwise conv + GELU), see profiles/pmc_poisson_n23_b8192.md"},
     }
     if not args.no_mixed:
         res["mixed_walkers"] = mixed_walkers(models_txt)

@@ -265,21 +265,23 @@ void drw_server::loop()
             inflight_peak = std::max<uint64_t>(inflight_peak, (uint64_t)busy);
         }
         if (progressed) continue;
+        if (busy > 0) {
+            // batches in flight: keep polling their events (the dispatcher owns
+            // a core while work is on the device; completion latency is the
+            // serving latency)
+            std::this_thread::yield();
+            continue;
+        }
         std::unique_lock<std::mutex> lk(mu);
         bool pending = false;
         for (auto& qd : queue) pending |= !qd.empty();
-        if (stop && !pending && busy == 0) return;
-        if (busy > 0) {
-            // batches in flight: poll their events again shortly
-            cv_work.wait_for(lk, std::chrono::microseconds(20));
-        } else {
-            cv_work.wait(lk, [&] {
-                if (stop) return true;
-                for (auto& qd : queue)
-                    if (!qd.empty()) return true;
-                return false;
-            });
-        }
+        if (stop && !pending) return;
+        cv_work.wait(lk, [&] {
+            if (stop) return true;
+            for (auto& qd : queue)
+                if (!qd.empty()) return true;
+            return false;
+        });
     }
 }
 

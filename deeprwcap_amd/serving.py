@@ -48,6 +48,16 @@ class Server:
                                            ctypes.byref(h)), "drw_server_create")
         self._h = h
 
+    @classmethod
+    def from_tuning(cls, path, device=0, models_txt=None, **kw):
+        """Server with the batch shape chosen by deeprwcap_amd.autotune."""
+        import json
+        with open(path) as f:
+            t = json.load(f)
+        kw.setdefault("max_batch", int(t["max_batch"]))
+        kw.setdefault("max_inflight", int(t.get("max_inflight", 4)))
+        return cls(device, models_txt, **kw)
+
     def submit(self, kind, N, axis, layer, cuboid, qid=0) -> int:
         L = np.ascontiguousarray(layer, np.float32).reshape(-1)
         C = np.ascontiguousarray(cuboid, np.float32).reshape(-1)
