@@ -45,18 +45,20 @@ def models_txt_or_random(path):
     return W.export_model_dir(d, {n: W.random_state(n, i) for i, n in enumerate(W.MODEL_ORDER)}), "random"
 
 
-def measured_traffic(batch):
-    """HBM bytes per launch of face_query_kernel<0> from the committed
-    rocprofv3 counter summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
-    calibration), for the batch it was measured at; None otherwise."""
+def face_counters(batch):
+    """Committed rocprofv3 counter summary of face_query_kernel<0> at the
+    batch it was measured at (None otherwise): HBM bytes per launch
+    (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel calibration) and the
+    share of SIMD issue cycles its VALU instructions take."""
     path = os.path.join(ROOT, "profiles", "face_query_traffic.json")
     if not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
         t = json.load(f)
     if t.get("batch") != batch:
-        return None
-    return 1024.0 * (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"])
+        return None, None
+    traffic = 1024.0 * (t["fetch_kib"] * t["fetch_scale"] + t["write_kib"] * t["write_scale"])
+    return traffic, t.get("valu_busy")
 
 
 def stage_times(solver):
@@ -288,7 +290,7 @@ def main():
     lat = np.array(lat[20:]) * 1e3
 
     achieved = B * PRED_FLOP / (face_ms * 1e-3) / 1e12
-    traffic = measured_traffic(B)
+    traffic, valu_busy = face_counters(B)
     res = {
         "metric": "transition queries/s at N=23 (bf16 NN vs FP32 FDM, <=1e-2 L1); p99 latency",
         "value": value,
@@ -314,7 +316,7 @@ def main():
                                 "face_predictor": face_ms},
         "roofline": {"kernel": "face_query_kernel<0> (PoissonPredictor)", "bound": "mfma", "achieved": achieved,
                      "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP16_PEAK_TFLOPS,
-                     "traffic": traffic,
+                     "traffic": traffic, "valu_issue_frac": valu_busy,
                      "note": "algorithmic 1,292,876 FLOP per query; the kernel is bound by VALU issue "
                              "(depthwise conv + GELU), see profiles/pmc_poisson_n23_b8192.md"},
     }

@@ -73,6 +73,11 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise DrwError(f"{LIB_PATH} is missing: build it with `python -m deeprwcap_amd.build`")
+        # torch ships its own HIP runtime (same soname, libamdhip64.so.7) and
+        # asks for it by file name, so it only shares a runtime with this
+        # library when it is loaded first; two runtimes in one process leave
+        # torch without a GPU.  Load torch before the library.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

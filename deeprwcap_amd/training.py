@@ -125,8 +125,8 @@ def fdm_dataset(n: int, seed: int, tol: float = 1e-10, max_boxes: int = 2, chunk
     return cubes, G, Gz
 
 
-def _index_rot(cubes, f, kind):
-    """out[z][y][x] = in[src(z,y,x)] by explicit index tables (exact)."""
+def _rot_table(f, kind):
+    """Flat source index of out[z][y][x] for face f's rotation (exact)."""
     n = NET_N
     z, y, x = torch.meshgrid(torch.arange(n), torch.arange(n), torch.arange(n), indexing="ij")
     if kind == "poisson":
@@ -135,7 +135,12 @@ def _index_rot(cubes, f, kind):
     else:
         src = {0: (n - 1 - z, y, x), 1: (z, y, x), 2: (z, y, x), 3: (z, n - 1 - y, x), 4: (z, x, y),
                5: (z, x, n - 1 - y)}[f]
-    idx = (src[0] * n * n + src[1] * n + src[2]).reshape(-1).to(cubes.device)
+    return (src[0] * n * n + src[1] * n + src[2]).reshape(-1)
+
+
+def _index_rot(cubes, f, kind):
+    """out[z][y][x] = in[src(z,y,x)] for one face f."""
+    idx = _rot_table(f, kind).to(cubes.device)
     return cubes.reshape(cubes.shape[0], -1)[:, idx].reshape(cubes.shape)
 
 
@@ -173,7 +178,14 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
     avg = torch.optim.swa_utils.AveragedModel(
         model, multi_avg_fn=torch.optim.swa_utils.get_ema_multi_avg_fn(ema), use_buffers=True) if ema > 0 else None
 
-    def make_batch(idx, f):
+    # one rotation table per face the network is trained on; every sample of
+    # a batch draws its own face, so batch-norm statistics in training are
+    # those of the face mix the running statistics describe at inference
+    kind_rot = "poisson" if name == "PoissonPredictor" else "gradient"
+    rot = torch.stack([_rot_table(f, kind_rot) for f in faces]).to(dev) if faces[0] is not None else None
+    face_t = torch.tensor([f if f is not None else -1 for f in faces], device=dev)
+
+    def make_batch(idx, f=None):
         x = Xall[idx].float()
         if name == "PoissonSelector":
             return x, Gp[idx].sum(-1)
@@ -181,13 +193,16 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
             tot = Gg[idx].abs().sum(-1)
             s = tot.sum(-1, keepdim=True)
             return x, torch.cat([tot / s, s], 1)
+        j = torch.randint(0, len(faces), (idx.shape[0],), device=dev)
+        fb = face_t[j]
+        xr = torch.gather(x.reshape(x.shape[0], -1), 1, rot[j]).reshape(x.shape)
         if name == "PoissonPredictor":
-            k = Gp[idx, f]
-            return _index_rot(x, f, "poisson"), k / k.sum(-1, keepdim=True)
-        # gradient predictors: target face of the rotated cube
-        k = Gg[idx, f]
-        k = -k if f == 0 else k  # flip z of an odd-N cube negates the z-gradient kernel
-        return _index_rot(x, f, "gradient"), k / k.abs().sum(-1, keepdim=True)
+            k = Gp[idx, fb]
+            return xr, k / k.sum(-1, keepdim=True)
+        # gradient predictors: target face of the rotated cube; flipping z of
+        # an odd-N cube negates the z-gradient kernel (face 0)
+        k = Gg[idx, fb] * torch.where(fb == 0, -1.0, 1.0)[:, None]
+        return xr, k / k.abs().sum(-1, keepdim=True)
 
     def loss_of(x, tgt):
         out = model(x)
@@ -230,9 +245,8 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         return e
 
     def step(st, total):
-        f = faces[st % len(faces)]
         idx = torch.randint(0, ntr, (batch,), device=dev)
-        x, tgt = make_batch(idx, f)
+        x, tgt = make_batch(idx)
         loss = loss_of(x, tgt)
         opt.zero_grad(set_to_none=True)
         loss.backward()
