@@ -172,6 +172,30 @@ def capacitance_check(models_txt, walks):
     return out
 
 
+def cap3d_check(models_txt, path, walks):
+    """Total capacitance of the master net of a cap3d structure by the walker,
+    with network queries and with FDM-only queries (N = 23), against the
+    reference's field-solver result (ref .dspf of the same name)."""
+    from deeprwcap_amd import cap3d
+    from deeprwcap_amd.serving import Server
+    s = cap3d.read(path)
+    dspf = os.path.splitext(path)[0] + ".dspf"
+    net, ref = cap3d.read_reference_total(open(dspf).read()) if os.path.exists(dspf) else (s["masters"][0], None)
+    out = {"structure": os.path.basename(path), "net": net, "reference_F": ref, "walks": walks}
+    for name, mt in (("network", models_txt), ("fdm", None)):
+        sv = Server(0, mt, max_batch=4096, max_inflight=4, hbm_budget_mb=8192, fdm_tol=1e-6)
+        t0 = time.perf_counter()
+        c, e, row, st = cap3d.master_capacitance(sv, s, net, walks=walks)
+        dt = time.perf_counter() - t0
+        sv.close()
+        out[name] = {"C_F": c, "stderr_F": e, "seconds": dt, "hops_per_walk": st["hops_per_walk"],
+                     "vs_reference": None if ref is None else c / ref - 1.0}
+        log(f"{name}: C {c:.5g} F +- {e:.2g} ({dt:.1f} s, {st['hops_per_walk']:.1f} hops/walk)"
+            + ("" if ref is None else f", reference {ref:.5g} F ({100 * (c / ref - 1):+.2f}%)"))
+    out["network_vs_fdm"] = out["network"]["C_F"] / out["fdm"]["C_F"] - 1.0
+    return out
+
+
 def serving_runs(models_txt, rates, duration_s=2.0, threads=64):
     """Serving loop: 64 host threads issue mixed Poisson/gradient N = 23
     queries with Poisson arrivals; returns per-rate latency and throughput."""
@@ -211,6 +235,10 @@ def main():
     ap.add_argument("--capacitance", type=int, default=0, metavar="WALKS",
                     help="only run the end-to-end check: capacitance row of the 2x8-wire structure from network "
                          "queries vs FDM-only queries (N = 23), WALKS walks each")
+    ap.add_argument("--cap3d", metavar="PATH", help="only run the end-to-end check on a cap3d structure: total "
+                    "capacitance of its master net from network queries and from FDM-only queries, beside the "
+                    "field-solver value in the .dspf next to it")
+    ap.add_argument("--walks", type=int, default=20000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -231,6 +259,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copied 3 x 1 GiB")
+        return
+    if args.cap3d:
+        models_txt, wkind = models_txt_or_random(args.models)
+        print(json.dumps(cap3d_check(models_txt, args.cap3d, args.walks)), flush=True)
         return
     if args.capacitance:
         models_txt, wkind = models_txt_or_random(args.models)

@@ -37,8 +37,10 @@ namespace {
 struct Geo {
     double win[6];
     std::vector<double> lz, leps;  // layer bottoms (ascending) and permittivities
-    std::vector<double> box;       // [n][6]
+    std::vector<double> box;       // [n][6] conductor boxes
     std::vector<int> id;           // conductor id per box
+    std::vector<double> mbox;      // [m][6] dielectric boxes (override the layers, first listed wins)
+    std::vector<double> meps;      // [m]
     int ncond = 0;
 };
 
@@ -61,7 +63,38 @@ double eps_at(const Geo& g, double z)
     return g.leps[i];
 }
 
-// Structure description of the cube centred at c with half side h.
+// Largest half side <= h for which the cube centred at c crosses at most
+// kMaxLayers - 1 layer interfaces and overlaps at most kMaxCuboids
+// dielectric boxes (the limits of a query's structure description,
+// dnn_solver.h:28-29): h drops to the distance of the first element past the
+// limit, which then only touches the cube.
+double fit_cube(const Geo& g, const double* c, double h, std::vector<double>& tmp)
+{
+    tmp.clear();
+    for (size_t k = 1; k < g.lz.size(); k++) {
+        const double d = std::fabs(g.lz[k] - c[2]);
+        if (d < h) tmp.push_back(d);
+    }
+    if ((int)tmp.size() > drw::kMaxLayers - 1) {
+        std::nth_element(tmp.begin(), tmp.begin() + (drw::kMaxLayers - 1), tmp.end());
+        h = tmp[drw::kMaxLayers - 1];
+    }
+    tmp.clear();
+    for (size_t i = 0; i < g.meps.size(); i++) {
+        const double d = box_dist(&g.mbox[6 * i], c);
+        if (d < h) tmp.push_back(d);
+    }
+    if ((int)tmp.size() > drw::kMaxCuboids) {
+        std::nth_element(tmp.begin(), tmp.begin() + drw::kMaxCuboids, tmp.end());
+        h = tmp[drw::kMaxCuboids];
+    }
+    return h;
+}
+
+// Structure description of the cube centred at c with half side h, in the
+// cube's normalised coordinates (dnn_solver.h:27-30): the layers it crosses
+// and the dielectric boxes it overlaps (first listed wins, as in the
+// voxelizer).
 void describe(const Geo& g, const double* c, double h, float* layer, float* cub)
 {
     for (int i = 0; i < 30; i++) layer[i] = drw::kPad;
@@ -77,6 +110,17 @@ void describe(const Geo& g, const double* c, double h, float* layer, float* cub)
             layer[2 * n + 1] = (float)g.leps[k];
             n++;
         }
+    }
+    int m = 0;
+    for (size_t i = 0; i < g.meps.size() && m < drw::kMaxCuboids; i++) {
+        const double* b = &g.mbox[6 * i];
+        if (!(box_dist(b, c) < h)) continue;
+        for (int a = 0; a < 3; a++) {
+            cub[7 * m + a] = (float)std::max((b[a] - c[a]) / h, -1.5);
+            cub[7 * m + 3 + a] = (float)std::min((b[a + 3] - c[a]) / h, 1.5);
+        }
+        cub[7 * m + 6] = (float)g.meps[i];
+        m++;
     }
 }
 
@@ -102,6 +146,11 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
     g.leps.assign(geo->layer_eps, geo->layer_eps + geo->n_layers);
     g.box.assign(geo->cond_box, geo->cond_box + 6 * geo->n_boxes);
     g.id.assign(geo->cond_id, geo->cond_id + geo->n_boxes);
+    if (geo->n_media > 0) {
+        if (!geo->media_box || !geo->media_eps) return drw::set_error(DRW_EINVAL, "walk: media arrays missing");
+        g.mbox.assign(geo->media_box, geo->media_box + 6 * geo->n_media);
+        g.meps.assign(geo->media_eps, geo->media_eps + geo->n_media);
+    }
     for (int v : g.id) g.ncond = std::max(g.ncond, v + 1);
     if (geo->n_layers < 1 || geo->n_boxes < 1 || master < 0 || master >= g.ncond)
         return drw::set_error(DRW_EINVAL, "walk: bad geometry or master");
@@ -185,6 +234,7 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
         std::vector<float> LL((size_t)W * 30), CC((size_t)W * 105), outs((size_t)W * 5);
         std::vector<uint64_t> tk(W);
         std::vector<int> idx;
+        std::vector<double> tmp;
         for (;;) {
             idx.clear();
             for (int i = 0; i < W; i++) {
@@ -208,6 +258,7 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
                 }
                 double h = std::min(std::min(dc, dw), wc->max_cube);
                 if (w.first) h = std::min(h, margin);
+                h = fit_cube(g, w.p, h, tmp);
                 if (!(h > 0)) {  // a start point on a conductor or the window: geometry error
                     std::lock_guard<std::mutex> lk(mu);
                     if (first_bad.empty()) {

@@ -28,7 +28,8 @@ class LoadConfig(ctypes.Structure):
 class Geometry(ctypes.Structure):
     _fields_ = [("window", ctypes.c_double * 6), ("n_layers", ctypes.c_int), ("layer_z", ctypes.c_void_p),
                 ("layer_eps", ctypes.c_void_p), ("n_boxes", ctypes.c_int), ("cond_box", ctypes.c_void_p),
-                ("cond_id", ctypes.c_void_p)]
+                ("cond_id", ctypes.c_void_p), ("n_media", ctypes.c_int), ("media_box", ctypes.c_void_p),
+                ("media_eps", ctypes.c_void_p)]
 
 
 class WalkConfig(ctypes.Structure):
@@ -121,15 +122,18 @@ class Server:
                     stop_dist=None, gauss_frac=0.5, max_cube=None, seed=1):
         """FRW capacitance row C[master][:] of a z-layered structure.
 
-        geometry: window (6,), layer_z (n,), layer_eps (n,), boxes (m, 6), ids (m,)."""
+        geometry: window (6,), layer_z (n,), layer_eps (n,), boxes (m, 6), ids (m,),
+        optionally media_boxes (k, 6) and media_eps (k,) (dielectric boxes)."""
         lz = np.ascontiguousarray(geometry["layer_z"], np.float64)
         le = np.ascontiguousarray(geometry["layer_eps"], np.float64)
         bx = np.ascontiguousarray(geometry["boxes"], np.float64)
         ids = np.ascontiguousarray(geometry["ids"], np.int32)
+        mb = np.ascontiguousarray(geometry.get("media_boxes", np.zeros((0, 6))), np.float64).reshape(-1, 6)
+        me = np.ascontiguousarray(geometry.get("media_eps", np.zeros(0)), np.float64)
         win = np.asarray(geometry["window"], np.float64)
         scale = float(np.min(bx[:, 3:] - bx[:, :3]))
         g = Geometry((ctypes.c_double * 6)(*win), len(lz), lz.ctypes.data, le.ctypes.data, len(ids), bx.ctypes.data,
-                     ids.ctypes.data)
+                     ids.ctypes.data, len(me), mb.ctypes.data if len(me) else None, me.ctypes.data if len(me) else None)
         wc = WalkConfig(walks, threads, concurrent, N, stop_dist or 1e-3 * scale, gauss_frac,
                         max_cube or float(np.max(win[3:] - win[:3])), seed)
         nc = int(ids.max()) + 1

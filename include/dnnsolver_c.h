@@ -227,8 +227,8 @@ int drw_server_run_load(drw_server* sv, const drw_load_config* load, const float
 /* ------------------------------------------------------------------------
  * Floating-random-walk capacitance extraction (the consumer of the query
  * path; stands in for the closed walker executable/bin/deepRWCap).  Planar
- * z-layers of dielectric, conductors as unions of boxes, Dirichlet 0 on the
- * window.  Walks start on a Gaussian surface around `master`, take one
+ * z-layers of dielectric, optional dielectric boxes over them, conductors
+ * as unions of boxes, Dirichlet 0 on the window.  Walks start on a Gaussian surface around `master`, take one
  * gradient query then Poisson queries through `sv` until they come within
  * stop_dist of a conductor; cap[k] receives C[master][k] (the units of the
  * coordinates times eps; multiply by eps0 for farads), stderr_out[k] its
@@ -243,6 +243,9 @@ typedef struct {
     int n_boxes;
     const double* cond_box;    /* [n_boxes][6] */
     const int32_t* cond_id;    /* [n_boxes] conductor index of each box */
+    int n_media;               /* dielectric boxes overriding the layers (0: none) */
+    const double* media_box;   /* [n_media][6] */
+    const double* media_eps;   /* [n_media] */
 } drw_geometry;
 typedef struct {
     int walks;                 /* total walks */
