@@ -81,36 +81,33 @@ __device__ __forceinline__ float vox_value(const VoxDesc& d, int z, int y, int x
 // Index remaps, all as out[z][y][x] = in[src] (cuda_kernels.cu:299-486 and
 // the rot90 of dnn_solver_gradient.cpp:228-238).
 // ------------------------------------------------------------------------
+// Written as selects (no switch) so that the coordinates stay in registers
+// whatever the face / axis.
 __device__ __forceinline__ void remap_poisson(int f, int& z, int& y, int& x)
 {
     const int Z = z, Y = y, X = x;
-    switch (f) {
-    case 1: z = kN - 1 - Z; break;
-    case 2: z = Y; y = Z; break;
-    case 3: z = Y; y = kN - 1 - Z; break;
-    case 4: z = Y; y = X; x = Z; break;
-    case 5: z = Y; y = X; x = kN - 1 - Z; break;
-    default: break;
-    }
+    // f: 0 identity; 1 (22-Z, Y, X); 2 (Y, Z, X); 3 (Y, 22-Z, X); 4 (Y, X, Z); 5 (Y, X, 22-Z)
+    z = f == 0 ? Z : f == 1 ? kN - 1 - Z : Y;
+    y = f <= 1 ? Y : f == 2 ? Z : f == 3 ? kN - 1 - Z : X;
+    x = f <= 3 ? X : f == 4 ? Z : kN - 1 - Z;
 }
 
 __device__ __forceinline__ void remap_gradient(int f, int& z, int& y, int& x)
 {
     const int Z = z, Y = y, X = x;
-    switch (f) {
-    case 0: z = kN - 1 - Z; break;
-    case 3: y = kN - 1 - Y; break;
-    case 4: y = X; x = Y; break;
-    case 5: y = X; x = kN - 1 - Y; break;
-    default: break;
-    }
+    // f: 0 (22-Z, Y, X); 1, 2 identity; 3 (Z, 22-Y, X); 4 (Z, X, Y); 5 (Z, X, 22-Y)
+    z = f == 0 ? kN - 1 - Z : Z;
+    y = f == 3 ? kN - 1 - Y : f >= 4 ? X : Y;
+    x = f == 4 ? Y : f == 5 ? kN - 1 - Y : X;
 }
 
 __device__ __forceinline__ void remap_axis(int axis, int& z, int& y, int& x)
 {
     const int Z = z, Y = y, X = x;
-    if (axis == 0) { z = kN - 1 - X; x = Z; }
-    else if (axis == 1) { z = kN - 1 - Y; y = Z; }
+    // axis 0: (22-X, Y, Z); axis 1: (22-Y, Z, X); axis 2: identity
+    z = axis == 0 ? kN - 1 - X : axis == 1 ? kN - 1 - Y : Z;
+    y = axis == 1 ? Z : Y;
+    x = axis == 0 ? Z : X;
 }
 
 // ------------------------------------------------------------------------

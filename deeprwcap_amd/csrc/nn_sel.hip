@@ -17,102 +17,149 @@
 
 namespace drw {
 
-// Conv3d(CIN -> COUT, k3, s2, no padding) over in[CIN][DIN^3] (fp16) as an
-// implicit GEMM; out[COUT][DOUT^3] fp16 after bias + GELU, or fp32 features.
-// koff (LDS, >= KT*16 ints) receives the im2col offset of every k = ci*27 +
-// tap (-1 past K), so a gathered element costs one add and one LDS read.
-// With COUT < 16 only COUT of the 16 MFMA columns hold outputs; the valid
-// ones are first moved across lanes (ds_bpermute) so that every lane
-// evaluates 16*COUT/64 GELUs instead of 4.
-template <int CIN, int COUT, int DIN, int DOUT, bool FP32_OUT>
+// Conv3d(CIN -> COUT, k3, s2, no padding) over in[ci][z][y][x] (fp16,
+// strides SC, SZ, SY, 1) as an implicit GEMM (rows = output positions,
+// k = ci*27 + tap) with bias + GELU.  koff (LDS, >= KT*16 ints) receives the
+// im2col offset of every k (-1 past K), so a gathered element costs one add
+// and one LDS read.  With COUT < 16 only COUT of the 16 MFMA columns hold
+// outputs; the valid ones are first moved across lanes (ds_bpermute) so that
+// every lane evaluates 16*COUT/64 GELUs instead of 4.
+//
+// Output layouts:
+// * SLABS: fp16 out[oz][co][oy*DOUT+ox], one output z-slab per phase with a
+//   barrier after it.  Slab oz reads input planes 2oz..2oz+2, so after its
+//   barrier the planes below 2oz+2 are dead and the slab's outputs (held in
+//   registers until then) are written over them: `out` may alias `in`
+//   (requires COUT * DOUT^2 <= 2 SZ).
+// * otherwise out[co][oz*DOUT^2+oy*DOUT+ox], fp16 or fp32 (FP32_OUT); `out`
+//   must not overlap `in`.
+template <int CIN, int COUT, int DIN, int DOUT, int SC, int SZ, int SY, bool SLABS, bool FP32_OUT>
 __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, const float* bias, int* koff)
 {
-    constexpr int M = DOUT * DOUT * DOUT, MT = (M + 15) / 16;
+    constexpr int DO2 = DOUT * DOUT, M = DOUT * DO2;
     constexpr int K = CIN * 27, KT = (K + 15) / 16;
     constexpr int NT = (COUT + 15) / 16;
-    constexpr int D2 = DIN * DIN, DO2 = DOUT * DOUT;
+    constexpr int VPT = COUT < 16 ? 16 * COUT / 64 : 4 * NT;  // outputs per lane per tile
+    constexpr int MP = SLABS ? DO2 : M;                      // positions per phase
+    constexpr int TP = (MP + 15) / 16, TPW = (TP + 3) / 4;   // tiles per phase / per wave
+    static_assert(COUT >= 16 || (16 * COUT) % 64 == 0, "compacted tiles need COUT a multiple of 4");
+    static_assert(!SLABS || (COUT * DO2 <= 2 * SZ && !FP32_OUT), "slab outputs must fit the freed planes");
     for (int k = threadIdx.x; k < KT * 16; k += 256) {
         int o = -1;
         if (k < K) {
             const int ci = k / 27, tap = k - 27 * ci;
-            o = ci * DIN * D2 + (tap / 9) * D2 + ((tap / 3) % 3) * DIN + tap % 3;
+            o = ci * SC + (tap / 9) * SZ + ((tap / 3) % 3) * SY + tap % 3;
         }
         koff[k] = o;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = 4 * (lane >> 4);
-    for (int mt = wave; mt < MT; mt += 4) {
-        const int m = mt * 16 + r;
-        int base = -1;
-        if (m < M) {
-            const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
-            base = 2 * oz * D2 + 2 * oy * DIN + 2 * ox;
-        }
-        float4_t acc[NT];
+    half4_t bw[NT][KT > 2 ? 1 : KT];  // fragments of short K kept in registers
+    if constexpr (KT <= 2) {
 #pragma unroll
-        for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
-        for (int kt = 0; kt < KT; kt++) {
-            const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
-            const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
-            half4_t a;
+        for (int n = 0; n < NT; n++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
-#pragma unroll
-            for (int n = 0; n < NT; n++) acc[n] = mfma16(a, frag[(n * KT + kt) * 64 + lane], acc[n]);
-        }
+            for (int kt = 0; kt < KT; kt++) bw[n][kt] = frag[(n * KT + kt) * 64 + lane];
+    }
+    // (channel, position within the phase) of value j of tile t held by this lane
+    auto where = [&](int t, int j, int& ch, int& mm) {
         if constexpr (COUT < 16) {
-            // output o = lane*E + e  ->  (row, col) = (o / COUT, o % COUT), held by
-            // lane col + 16*(row/4) as element row % 4
-            constexpr int E = 16 * COUT / 64;
+            const int o = lane * VPT + j;
+            ch = o % COUT;
+            mm = t * 16 + o / COUT;
+        } else {
+            ch = (j >> 2) * 16 + r;
+            mm = t * 16 + kq + (j & 3);
+        }
+    };
+    constexpr int PHASES = SLABS ? DOUT : 1;
+    for (int ph = 0; ph < PHASES; ph++) {
+        float held[TPW * VPT];
 #pragma unroll
-            for (int e = 0; e < E; e++) {
-                const int o = lane * E + e, row = o / COUT, col = o % COUT;
-                const int src = col + 16 * (row >> 2), el = row & 3;
-                float v = 0.f;
+        for (int u = 0; u < TPW; u++) {
+            const int t = wave + 4 * u;
+            if (t >= TP) continue;
+            const int mloc = t * 16 + r, m = ph * MP + mloc;
+            int base = -1;
+            if (mloc < MP) {
+                const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
+                base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
+            }
+            float4_t acc[NT];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const float t = __shfl(acc[0][i], src, 64);
-                    v = el == i ? t : v;
-                }
-                const int mm = mt * 16 + row;
-                if (mm < M) {
-                    const float g = gelu(v + bias[col]);
-                    if (FP32_OUT) ((float*)outp)[col * M + mm] = g;
-                    else ((half_t*)outp)[col * M + mm] = (half_t)g;
+            for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
+            for (int kt = 0; kt < KT; kt++) {
+                const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
+                const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
+                half4_t a;
+#pragma unroll
+                for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    half4_t b;
+                    if constexpr (KT <= 2) b = bw[n][kt];
+                    else b = frag[(n * KT + kt) * 64 + lane];
+                    acc[n] = mfma16(a, b, acc[n]);
                 }
             }
-        } else {
 #pragma unroll
-            for (int n = 0; n < NT; n++) {
-                const int ch = n * 16 + r;
-                const float bb = bias[ch];
+            for (int j = 0; j < VPT; j++) {
+                float v;
+                int ch, mm;
+                where(t, j, ch, mm);
+                if constexpr (COUT < 16) {
+                    // output o = lane*VPT + j -> (row, col) = (o / COUT, o % COUT),
+                    // held by lane col + 16*(row/4) as element row % 4
+                    const int row = (lane * VPT + j) / COUT;
+                    const int src = ch + 16 * (row >> 2), el = row & 3;
+                    v = 0.f;
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int mm = mt * 16 + kq + i;
-                    if (mm >= M) continue;
-                    const float v = gelu(acc[n][i] + bb);
-                    if (FP32_OUT) ((float*)outp)[ch * M + mm] = v;
-                    else ((half_t*)outp)[ch * M + mm] = (half_t)v;
+                    for (int i = 0; i < 4; i++) {
+                        const float tv = __shfl(acc[0][i], src, 64);
+                        v = el == i ? tv : v;
+                    }
+                } else {
+                    v = acc[j >> 2][j & 3];
+                }
+                const float g = gelu(v + bias[ch]);
+                if constexpr (SLABS) {
+                    held[u * VPT + j] = g;
+                } else if (mm < MP) {
+                    if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
+                    else ((half_t*)outp)[ch * M + mm] = (half_t)g;
+                }
+            }
+        }
+        __syncthreads();  // this phase's inputs fully read
+        if constexpr (SLABS) {
+            half_t* o = (half_t*)outp + ph * (COUT * DO2);
+#pragma unroll
+            for (int u = 0; u < TPW; u++) {
+                const int t = wave + 4 * u;
+                if (t >= TP) continue;
+#pragma unroll
+                for (int j = 0; j < VPT; j++) {
+                    int ch, mm;
+                    where(t, j, ch, mm);
+                    if (mm < MP) o[ch * DO2 + mm] = (half_t)held[u * VPT + j];
                 }
             }
         }
     }
-    __syncthreads();  // koff is rewritten by the next layer
+    if constexpr (SLABS) __syncthreads();  // outputs visible; koff free for the next layer
 }
 
-// LDS of one query, sized per architecture (FaceSelector: C1 = 4, C2 = 8,
-// 36.9 KB; FaceSelectorWeight: C1 = 8, C2 = 16, 47.6 KB).  The second conv
-// output reuses the cube's storage: the cube is dead once the first conv has
-// finished (conv3d_mfma ends with a barrier).
+// LDS of one query: the cube, the first conv's output (written slab by slab
+// over the cube planes it no longer needs, slab-major [oz][c][11*11]) and
+// the second conv's output ([c][125], after the first's) share one region:
+// 26.4 KB in all for FaceSelector (6 workgroups per CU), 27.4 KB for
+// FaceSelectorWeight (5 per CU).
 template <int C1, int C2>
 struct SelSmem {
+    static constexpr int kRegion = kNNN > C1 * 1331 + C2 * 125 ? kNNN : C1 * 1331 + C2 * 125;
     alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4)
-    union {
-        half_t cube[kNNN];  // 24,334 B
-        half_t l2[C2 * 125];
-    };
-    half_t l1[C1 * 1331];
+    half_t cube[kRegion];           // >= 24,334 B
     VoxDesc desc;
 };
 
@@ -120,7 +167,7 @@ template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs a, SelNetDev w)
 {
     constexpr int C1 = WEIGHTED ? 8 : 4, C2 = WEIGHTED ? 16 : 8, C3 = WEIGHTED ? 64 : 16;
-    static_assert(C2 * 125 <= kNNN, "second conv output must fit the cube's storage");
+    static_assert(C1 * 1331 + C2 * 125 <= SelSmem<C1, C2>::kRegion, "conv outputs must fit the shared region");
     __shared__ SelSmem<C1, C2> s;
     const int b = blockIdx.x;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
@@ -133,20 +180,23 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
     for (int it = threadIdx.x; it < kNN * 4; it += 256) {
         const int r = it >> 2, q = it & 3;  // row r = 23 z + y of the output frame
         const int zo = r / kN, yo = r - zo * kN;
-        int z0 = zo, y0 = yo, x0 = 0, z1 = zo, y1 = yo, x1 = 1;
-        if (a.axis) {
-            remap_axis(axis, z0, y0, x0);
-            remap_axis(axis, z1, y1, x1);
-        }
+        // source of output voxel (zo, yo, x) under remap_axis: axis 0 ->
+        // (22 - x, yo, zo), axis 1 -> (22 - yo, zo, x), none / 2 -> (zo, yo, x)
         const VoxDesc& d = s.desc;
         const uint16_t* vm;
-        int base, dir;
+        int base, dir, zfix;
         uint32_t fixedm;
-        const bool zvar = z1 != z0;
-        if (zvar) { vm = d.bz; base = z0; dir = z1 - z0; fixedm = (uint32_t)d.by[y0] & d.bx[x0]; }
-        else if (y1 != y0) { vm = d.by; base = y0; dir = y1 - y0; fixedm = (uint32_t)d.bz[z0] & d.bx[x0]; }
-        else { vm = d.bx; base = x0; dir = x1 - x0; fixedm = (uint32_t)d.bz[z0] & d.by[y0]; }
-        const float lfix = d.lz[z0];
+        const bool zvar = a.axis && axis == 0;
+        if (zvar) {
+            vm = d.bz; base = kN - 1; dir = -1; zfix = 0;
+            fixedm = (uint32_t)d.by[yo] & d.bx[zo];
+        } else {
+            const bool ax1 = a.axis && axis == 1;
+            const int zs = ax1 ? kN - 1 - yo : zo, ys = ax1 ? zo : yo;
+            vm = d.bx; base = 0; dir = 1; zfix = zs;
+            fixedm = (uint32_t)d.bz[zs] & d.by[ys];
+        }
+        const float lfix = d.lz[zfix];
         const int xa = 6 * q, xb = q == 3 ? kN : xa + 6;
         half_t* dst = s.cube + r * kN;
         for (int x = xa; x < xb; x++) {
@@ -156,9 +206,13 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
         }
     }
     __syncthreads();
-    conv3d_mfma<1, C1, 23, 11, false>(s.cube, s.l1, w.frag[0], w.bias[0], s.koff);
-    conv3d_mfma<C1, C2, 11, 5, false>(s.l1, s.l2, w.frag[1], w.bias[1], s.koff);
-    conv3d_mfma<C2, C3, 5, 2, true>(s.l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2], s.koff);
+    half_t* l1 = s.cube;                 // [11][C1][121], over the cube
+    half_t* l2 = s.cube + C1 * 1331;     // [C2][125], after l1
+    conv3d_mfma<1, C1, 23, 11, 0, kNN, kN, true, false>(s.cube, l1, w.frag[0], w.bias[0], s.koff);
+    conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false, false>(l1, l2, w.frag[1], w.bias[1], s.koff);
+    __syncthreads();
+    conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, false, true>(l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2],
+                                                       s.koff);
 }
 
 // Linear layer on MFMA for 16 rows: X[16][K] (fp16, LDS) -> Y[16][N].
