@@ -28,7 +28,18 @@ struct VoxDesc {
     float lz[kN];          // layer value at each z (normalised)
     float cv[kMaxCuboids]; // cuboid values (normalised)
     float maxv;            // per-query max eps (dnn_solver.cpp:274-282)
+    uint32_t lzs[kN];            // lz and cv split into fp16 pairs hi | lo << 16,
+    uint32_t cvs[kMaxCuboids];   // hi + lo = the fp32 value to ~2^-22 relative
 };
+
+// fp16 pair (hi | lo << 16) with hi = fp16(v), lo = fp16(v - hi): an MFMA
+// operand in two fp16 parts carries v to about fp32 precision.
+__device__ __forceinline__ uint32_t half_split(float v)
+{
+    const half_t hi = (half_t)v;
+    const half_t lo = (half_t)(v - (float)hi);
+    return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
+}
 
 // Builds the description of query b with one wave (lanes 0..63 of the
 // calling wave); other waves must not call it.  Normalises by the max.
@@ -64,9 +75,15 @@ __device__ inline void vox_desc_build(VoxDesc& d, const float* layers, const flo
             if (mzv == kPad) break;
             if (cc >= mzv) li = l; else break;
         }
-        d.lz[lane] = Ly[2 * li + 1] / m;
+        const float lv = Ly[2 * li + 1] / m;
+        d.lz[lane] = lv;
+        d.lzs[lane] = half_split(lv);
     }
-    if (lane < kMaxCuboids) d.cv[lane] = Cb[7 * lane + 6] / m;
+    if (lane < kMaxCuboids) {
+        const float cv = Cb[7 * lane + 6] / m;
+        d.cv[lane] = cv;
+        d.cvs[lane] = half_split(cv);
+    }
     if (lane == 0) d.maxv = m;
 }
 
@@ -108,6 +125,17 @@ __device__ __forceinline__ void remap_axis(int axis, int& z, int& y, int& x)
     z = axis == 0 ? kN - 1 - X : axis == 1 ? kN - 1 - Y : Z;
     y = axis == 1 ? Z : Y;
     x = axis == 0 ? Z : X;
+}
+
+// K-slot order of the face predictors' input 1x1 conv (25 channels: cube
+// slices 0..22, then the w and h position planes 23, 24).  In an MFMA
+// operand, lane group g = lane/16 holds slots q = 4 kt + j (kt < 2, j < 4).
+// Group g takes slices 6g..6g+5, so every lane builds six slices; group 3's
+// sixth slot is plane 23, group 0's seventh is plane 24, the rest are zero
+// (-1).  The host packs the weights in the same order.
+__host__ __device__ constexpr int input_slot_channel(int g, int q)
+{
+    return q < 6 ? 6 * g + q : (q == 6 && g == 0 ? kN + 1 : -1);
 }
 
 // ------------------------------------------------------------------------

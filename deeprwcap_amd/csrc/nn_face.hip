@@ -249,88 +249,118 @@ __device__ void head_conv(const float* in, float* hout, const float* w, float b)
 }
 
 // ------------------------------------------------------------ input layer
+// Source voxel (z, y, x) of input element T[c][h][w] of the face predictors'
+// first layer: mode 0: rotate_poisson_f(cube); mode 1:
+// rotate_gradient_f(axis_rotate(cube)); mode 2: the same, permuted so that
+// channels are y (SignedFacePredictor channel_dim = 1).
+template <int MODE>
+__device__ __forceinline__ void input_src(int face, int axis, int c, int h, int w, int& z, int& y, int& x)
+{
+    if (MODE == 2) { z = h; y = c; x = w; } else { z = c; y = h; x = w; }
+    if (MODE == 0) {
+        remap_poisson(face, z, y, x);
+    } else {
+        remap_gradient(face, z, y, x);
+        remap_axis(axis, z, y, x);
+    }
+}
+
 // First 1x1 conv (25 input channels -> COUT) with its operand built in
-// registers: the MFMA A fragment of lane l is pixel m*16 + l%16, channels
-// kt*16 + 4(l/16) + j, i.e. input element T[c][h][w] of
-// mode 0: rotate_poisson_f(cube); mode 1: rotate_gradient_f(axis_rotate(cube));
-// mode 2: the same, permuted so that channels are y (SignedFacePredictor
-// channel_dim = 1); channels 23, 24 are the w and h position planes, 25..31
-// zero.  Every remap is a permutation with flips, so along the 23 channels of
-// a pixel exactly one source axis varies (by +-1 per channel) and the cuboid
-// masks of the two fixed axes are AND-ed once per pixel.
+// registers: the MFMA B fragment of lane l is pixel m*16 + l%16, K slots
+// 4 kt + j of lane group l/16 in input_slot_channel order.  Every remap is a
+// permutation with flips, so the source voxel is an affine function of
+// (c, h, w) in which exactly one source axis moves with c, and its offset and
+// slopes depend on the query's face and axis only: the moving axis' cuboid
+// masks (and layer values, when it is z) of a lane's six slices are loaded
+// once, and a pixel only ANDs them with the masks of its two fixed axes.
+// The product runs in split precision, W a ~= Wh ah + Wh al + Wl ah with
+// fp16 parts of the fp32 operands: the input layer's rounding is otherwise
+// the largest share of the network's error against the fp32 forward.
 template <int MODE, int COUT, int CSO>
 __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out, const half4_t* frag,
-                              const float* bias, const float2* gtab)
+                              const half4_t* frag_lo, const float* bias, const float2* gtab)
 {
     static_assert(COUT % 16 == 0, "full MFMA column tiles only");
     constexpr int KT = 2, NT = COUT / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    half4_t bw[NT][KT];
+    const int g = lane >> 4, r = lane & 15;
+    half4_t bw[NT][KT], bl[NT][KT];
     float bb[NT][4];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
 #pragma unroll
-        for (int k = 0; k < KT; k++) bw[n][k] = frag[(n * KT + k) * 64 + lane];
-#pragma unroll
-        for (int i = 0; i < 4; i++) bb[n][i] = bias[n * 16 + 4 * (lane >> 4) + i];
-    }
-    auto src = [&](int c, int h, int w, int& z, int& y, int& x) {
-        if (MODE == 2) { z = h; y = c; x = w; } else { z = c; y = h; x = w; }
-        if (MODE == 0) {
-            remap_poisson(face, z, y, x);
-        } else {
-            remap_gradient(face, z, y, x);
-            remap_axis(axis, z, y, x);
+        for (int k = 0; k < KT; k++) {
+            bw[n][k] = frag[(n * KT + k) * 64 + lane];
+            bl[n][k] = frag_lo[(n * KT + k) * 64 + lane];
         }
-    };
-    const int r = lane & 15, kq = 4 * (lane >> 4);
+#pragma unroll
+        for (int i = 0; i < 4; i++) bb[n][i] = bias[n * 16 + 4 * g + i];
+    }
+    int oz, oy, ox, cz, cy, cx, hz, hy, hx, wz, wy, wx;
+    input_src<MODE>(face, axis, 0, 0, 0, oz, oy, ox);
+    input_src<MODE>(face, axis, 1, 0, 0, cz, cy, cx);
+    input_src<MODE>(face, axis, 0, 1, 0, hz, hy, hx);
+    input_src<MODE>(face, axis, 0, 0, 1, wz, wy, wx);
+    cz -= oz; cy -= oy; cx -= ox;
+    hz -= oz; hy -= oy; hx -= ox;
+    wz -= oz; wy -= oy; wx -= ox;
+    const bool zmov = cz != 0, ymov = cy != 0, xmov = cx != 0;
+    const int vbase = zmov ? oz : ymov ? oy : ox, vdir = cz + cy + cx;
+    const uint16_t* vm = zmov ? d.bz : ymov ? d.by : d.bx;
+    uint32_t vmask[6], vlz[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int c = min(6 * g + q, kN - 1);  // group 3's sixth slot is a position plane
+        const int k = vbase + vdir * c;
+        vmask[q] = vm[k];
+        vlz[q] = d.lzs[k];
+    }
     for (int m = wave; m < kPix / 16; m += 4) {
         const int p = m * 16 + r;
-        half4_t a[KT];
-        if (p < kNN) {
-            const int h = p / kN, w = p - h * kN;
-            int z0, y0, x0, z1, y1, x1;
-            src(0, h, w, z0, y0, x0);
-            src(1, h, w, z1, y1, x1);
-            const uint16_t* vm;
-            int base, dir;
-            uint32_t fixedm;
-            const bool zvar = z1 != z0;
-            if (zvar) { vm = d.bz; base = z0; dir = z1 - z0; fixedm = (uint32_t)d.by[y0] & d.bx[x0]; }
-            else if (y1 != y0) { vm = d.by; base = y0; dir = y1 - y0; fixedm = (uint32_t)d.bz[z0] & d.bx[x0]; }
-            else { vm = d.bx; base = x0; dir = x1 - x0; fixedm = (uint32_t)d.bz[z0] & d.by[y0]; }
+        const int pc = min(p, kNN - 1);  // rows past the last pixel are zeroed below
+        const int h = pc / kN, w = pc - h * kN;
+        // fixed-axis coordinates (the moving axis' value is in range and masked out)
+        const int zs = oz + hz * h + wz * w, ys = oy + hy * h + wy * w, xs = ox + hx * h + wx * w;
+        const uint32_t fixedm = (zmov ? 0xffffu : (uint32_t)d.bz[zs]) & (ymov ? 0xffffu : (uint32_t)d.by[ys]) &
+                                (xmov ? 0xffffu : (uint32_t)d.bx[xs]);
+        const uint32_t lzp = d.lzs[zs];
+        const bool live = p < kNN;
+        uint32_t v[8];  // fp16 pairs hi | lo << 16
 #pragma unroll
-            for (int kt = 0; kt < KT; kt++) {
+        for (int q = 0; q < 6; q++) {
+            const uint32_t mm = fixedm & vmask[q];
+            v[q] = mm ? d.cvs[__builtin_ctz(mm)] : (zmov ? vlz[q] : lzp);
+        }
+        // position planes (PosEmbed2D: linspace(0, 1, 23) over w, then h)
+        v[5] = g == 3 ? half_split((float)w * (1.0f / (kN - 1))) : v[5];
+        v[6] = g == 0 ? half_split((float)h * (1.0f / (kN - 1))) : 0u;
+        v[7] = 0u;
+        half4_t ah[KT], al[KT];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int c = kt * 16 + kq + j;
-                    float val = 0.f;
-                    if (c < kN) {
-                        const int k = base + dir * c;
-                        const uint32_t mm = fixedm & vm[k];
-                        val = mm ? d.cv[__builtin_ctz(mm)] : d.lz[zvar ? k : z0];
-                    } else if (c == kN) {
-                        val = (float)w / (float)(kN - 1);
-                    } else if (c == kN + 1) {
-                        val = (float)h / (float)(kN - 1);
-                    }
-                    a[kt][j] = (half_t)val;
-                }
-            }
-        } else {
+        for (int kt = 0; kt < KT; kt++) {
+            uint32_t e[4];
 #pragma unroll
-            for (int kt = 0; kt < KT; kt++) a[kt] = (half4_t){0, 0, 0, 0};
+            for (int j = 0; j < 4; j++) e[j] = live ? v[4 * kt + j] : 0u;
+            const uint2 hi2 = make_uint2((e[0] & 0xffffu) | (e[1] << 16), (e[2] & 0xffffu) | (e[3] << 16));
+            const uint2 lo2 = make_uint2((e[0] >> 16) | (e[1] & 0xffff0000u), (e[2] >> 16) | (e[3] & 0xffff0000u));
+            ah[kt] = __builtin_bit_cast(half4_t, hi2);
+            al[kt] = __builtin_bit_cast(half4_t, lo2);
         }
         // transposed product (see pw_mfma): 4 consecutive channels of pixel p
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
+            for (int k = 0; k < KT; k++) {
+                acc = mfma16(bl[n][k], ah[k], acc);
+                acc = mfma16(bw[n][k], al[k], acc);
+            }
+#pragma unroll
+            for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], ah[k], acc);
             half4_t o;
 #pragma unroll
             for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i] + bb[n][i], gtab);
-            *(half4_t*)(out + p * CSO + n * 16 + kq) = o;
+            *(half4_t*)(out + p * CSO + n * 16 + 4 * g) = o;
         }
     }
 }
@@ -356,7 +386,7 @@ struct FaceSmem {
 template <int MODE>
 __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0], s.gtab); __syncthreads();
+    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.frag0_lo, w.pw_bias[0], s.gtab); __syncthreads();
     dw_conv<16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
     pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.pw_bias[1], s.gtab); __syncthreads();
     dw_conv<16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
@@ -373,7 +403,7 @@ __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w
 
 __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.pw_bias[0], s.gtab); __syncthreads();
+    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.frag0_lo, w.pw_bias[0], s.gtab); __syncthreads();
     dw_conv<32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
     pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.pw_bias[1], s.gtab); __syncthreads();
     dw_conv<32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();

@@ -135,6 +135,24 @@ std::vector<half_t> pack_frag(const std::vector<float>& W, int N, int K)
     return out;
 }
 
+// The same for the face predictors' input layer (W [N][25]): K slots in the
+// kernel's operand order, input_slot_channel(lane group, 4 kt + j).
+std::vector<half_t> pack_frag_input(const std::vector<float>& W, int N)
+{
+    constexpr int K = 25, KT = 2;
+    const int NT = (N + 15) / 16;
+    std::vector<half_t> out((size_t)NT * KT * 64 * 4);
+    for (int n = 0; n < NT; n++)
+        for (int kt = 0; kt < KT; kt++)
+            for (int l = 0; l < 64; l++)
+                for (int j = 0; j < 4; j++) {
+                    const int col = n * 16 + (l & 15), ch = input_slot_channel(l >> 4, 4 * kt + j);
+                    const float v = (col < N && ch >= 0 && ch < K) ? W[(size_t)col * K + ch] : 0.f;
+                    out[(((size_t)n * KT + kt) * 64 + l) * 4 + j] = (half_t)v;
+                }
+    return out;
+}
+
 std::vector<float> pad(const std::vector<float>& v, size_t n)
 {
     std::vector<float> o(std::max(n, v.size()), 0.f);
@@ -165,7 +183,10 @@ int build_face(const TensorMap& m, const FaceArch& A, Arena& ar, FaceNetDev& d)
         std::vector<float> W(t->data);
         for (int o = 0; o < A.c0; o++)
             for (int i = 0; i < 25; i++) W[o * 25 + i] *= sc[o];
-        ar.add(pack_frag(W, A.c0, 25), (const half_t**)&d.frag[0]);
+        std::vector<float> Wlo(W.size());
+        for (size_t i = 0; i < W.size(); i++) Wlo[i] = W[i] - (float)(half_t)W[i];
+        ar.add(pack_frag_input(W, A.c0), (const half_t**)&d.frag[0]);
+        ar.add(pack_frag_input(Wlo, A.c0), (const half_t**)&d.frag0_lo);
         ar.add(pad(sh, 32), &d.pw_bias[0]);
     }
     for (size_t i = 0; i < A.cin.size(); i++) {
