@@ -31,6 +31,7 @@ PRED_FLOP = 2 * 646_438      # PoissonPredictor MACs per face evaluation (archit
 SEL_FLOP = 2 * 287_972       # PoissonSelector MACs per query (architecture.py:176-205)
 HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TFLOPS = 2500.0
+FP32_VALU_PEAK_TFLOPS = 157.3
 
 
 def log(msg):
@@ -79,6 +80,26 @@ def time_steps(run, steps, warmup, torch):
         run()
     torch.cuda.synchronize()
     return time.perf_counter() - t0
+
+
+def fdm_roofline(L, C, kernel_ms):
+    """Roofline of the batched FDM kernel on the same batch: the inner FP32
+    CG costs 25 N^3 FLOP per iteration (7-point SpMV 13, two dots 4, three
+    axpy 6, Jacobi 1, norm 1); the iteration counts K come from re-running
+    the (deterministic) solve with statistics on."""
+    from deeprwcap_amd import device as D
+    B, N = L.shape[0], 23
+    pk = D.PackedCubes([N] * B)
+    D.voxelize(L, C, fixed_N=N, normalise=False, out=pk.eps.view(B, N, N, N))
+    iters, _ = D.fdm_solve(pk, 0, tol=1e-6)
+    K = iters.cpu().numpy().astype(np.float64)
+    flop = 25.0 * N ** 3 * K.sum()
+    achieved = flop / (kernel_ms * 1e-3) / 1e12
+    return {"kernel": "fdm_pcg_kernel", "bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS, "iters_mean": float(K.mean()),
+            "hbm_algorithmic_bytes": B * (4 * N ** 3 + 48 * N ** 2 + 16),
+            "note": "K*25*N^3 FLOP per solve at tol 1e-6, fp32 vector peak; the kernel is bound by LDS and "
+                    "barrier latency (one 1024-thread workgroup per CU)"}
 
 
 def l1_vs_fdm(solver, L, C, n, torch):
@@ -306,7 +327,10 @@ def main():
     dt_f = time_steps(run_fdm, args.fdm_steps, 1, torch)
     fdm_rate = B * args.fdm_steps / dt_f
     fms, fcnt = stage_times(fdm)
-    log(f"FDM: {fdm_rate:,.0f} queries/s (FDM kernel {fms[4] / max(fcnt[4], 1):.3f} ms per launch)")
+    fdm_kernel_ms = fms[4] / max(fcnt[4], 1)
+    log(f"FDM: {fdm_rate:,.0f} queries/s (FDM kernel {fdm_kernel_ms:.3f} ms per launch)")
+    fdm_roof = fdm_roofline(L, C, fdm_kernel_ms)
+    log(f"FDM kernel: {fdm_roof['iters_mean']:.1f} CG iterations per cube, {fdm_roof['achieved']:.1f} TFLOP/s")
 
     # accuracy gate: network full kernels vs FP64 FDM
     l1_mean, l1_p99 = l1_vs_fdm(nn, L, C, 256, torch)
@@ -341,6 +365,7 @@ def main():
                    "weights": wkind, "parallelism": "single_gpu"},
         "fdm_queries_per_s": fdm_rate,
         "nn_over_fdm": value / fdm_rate,
+        "fdm_roofline": fdm_roof,
         "l1_vs_fp64_mean": l1_mean,
         "l1_vs_fp64_p99": l1_p99,
         "latency_ms_batch64": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},

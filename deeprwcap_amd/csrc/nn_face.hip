@@ -349,14 +349,16 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
         // transposed product (see pw_mfma): 4 consecutive channels of pixel p
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            float4_t acc = {0.f, 0.f, 0.f, 0.f};
+            // the small cross terms in a chain of their own, so the two
+            // dependent MFMA chains interleave
+            float4_t acc = {0.f, 0.f, 0.f, 0.f}, acl = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int k = 0; k < KT; k++) {
-                acc = mfma16(bl[n][k], ah[k], acc);
-                acc = mfma16(bw[n][k], al[k], acc);
+                acl = mfma16(bl[n][k], ah[k], acl);
+                acl = mfma16(bw[n][k], al[k], acl);
+                acc = mfma16(bw[n][k], ah[k], acc);
             }
-#pragma unroll
-            for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], ah[k], acc);
+            acc += acl;
             half4_t o;
 #pragma unroll
             for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i] + bb[n][i], gtab);
