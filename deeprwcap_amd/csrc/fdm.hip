@@ -15,7 +15,12 @@
 //    (N^3 + 2 N^2 floats with zero guard planes, so the stencil reads need no
 //    bounds checks: x/y wrap-around reads hit a zero weight); A p is parked
 //    in a third LDS array across the reduction so the per-thread state stays
-//    at 4 couplings + x + r per cell (no spills at 12 cells per thread);
+//    at 4 couplings + x + r per cell: the CG iteration loop runs without
+//    spills at 12 cells per thread (the set-up and FP64 residual passes
+//    spill a few registers to scratch).  The Chronopoulos-Gear form, one
+//    fused reduction per iteration, needs x, p and A p per cell besides r
+//    and spills inside the loop at 12 cells per thread, with no LDS left to
+//    park a vector in;
 //  * inner solver: FP32 CG on the scaled system (= Jacobi PCG), block-wide
 //    reductions by wave64 butterflies + a 16-entry LDS table, summed in a
 //    fixed order so every thread sees bit-identical alpha/beta;

@@ -8,6 +8,7 @@ namespace drw {
 
 typedef _Float16 half_t;
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kN = kNetN;            // 23
@@ -207,6 +208,22 @@ __device__ __forceinline__ float gelu_tab(float x, const float2* tab)
 __device__ __forceinline__ float4_t mfma16(half4_t a, half4_t b, float4_t c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
+}
+
+// MFMA 16x16x32 fp16 -> fp32, gfx950's double-rate shape: one instruction for
+// two K = 16 steps.  A and B spread K over the four lane groups in the same
+// way (8 values per lane), so a product only needs both operands to put the
+// same (k-slot) -> channel assignment in lane group l/16, element j; D is laid
+// out as for mfma16.
+__device__ __forceinline__ float4_t mfma32(half8_t a, half8_t b, float4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// Two half4 K-steps as one half8 operand (slots 0..3, then 4..7).
+__device__ __forceinline__ half8_t cat8(half4_t lo, half4_t hi)
+{
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 }  // namespace drw

@@ -52,8 +52,12 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
+            if constexpr (KT == 2) {
+                acc = mfma32(cat8(bw[n][0], bw[n][1]), cat8(a[0], a[1]), acc);
+            } else {
 #pragma unroll
-            for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
+                for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
+            }
             constexpr int VFULL = 16;
             const int V = COUT - n * 16 < VFULL ? COUT - n * 16 : VFULL;  // valid channels of this tile
             if (V == VFULL) {
@@ -349,15 +353,13 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
         // transposed product (see pw_mfma): 4 consecutive channels of pixel p
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            // the small cross terms in a chain of their own, so the two
-            // dependent MFMA chains interleave
-            float4_t acc = {0.f, 0.f, 0.f, 0.f}, acl = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int k = 0; k < KT; k++) {
-                acl = mfma16(bl[n][k], ah[k], acl);
-                acl = mfma16(bw[n][k], al[k], acl);
-                acc = mfma16(bw[n][k], ah[k], acc);
-            }
+            // the lane's eight K slots form one K = 32 operand; the small cross
+            // terms run in a chain of their own, so the two chains interleave
+            static_assert(KT == 2, "eight K slots per lane");
+            const half8_t a8 = cat8(ah[0], ah[1]), l8 = cat8(al[0], al[1]), w8 = cat8(bw[n][0], bw[n][1]);
+            float4_t acl = mfma32(cat8(bl[n][0], bl[n][1]), a8, (float4_t){0.f, 0.f, 0.f, 0.f});
+            acl = mfma32(w8, l8, acl);
+            float4_t acc = mfma32(w8, a8, (float4_t){0.f, 0.f, 0.f, 0.f});
             acc += acl;
             half4_t o;
 #pragma unroll

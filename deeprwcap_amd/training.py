@@ -151,13 +151,16 @@ def _kl(pred_logp, target):
 
 
 def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log=print, seed=0, minutes=None,
-              eval_every=500, n_eval=512, loss_kind="kl", ema=0.999):
+              eval_every=500, n_eval=512, loss_kind="kl", ema=0.999, init=None, best_path=None):
     """Train one network.  With `minutes`, the run is time-boxed: the number
     of steps is set from the measured step time so that the cosine schedule
     ends when the budget does.  An exponential moving average of the weights
     (decay `ema`, 0 = off) is what gets evaluated and exported.  Every
     `eval_every` steps the held-out error (last 5% of the cubes) is measured
-    and the best state is kept.  `loss_kind` for the PoissonPredictor: "kl"
+    and the best state is kept.  `init`: a .drwnet to start from (a warm
+    restart of the cosine schedule, as CosineAnnealingWarmRestarts does in
+    the reference training, train.py:38); `best_path`: a .drwnet rewritten
+    with every new best held-out state (a long run keeps its best so far).  `loss_kind` for the PoissonPredictor: "kl"
     (KL + 0.2 L1, the reference's main loss plus an L1 term) or "l1" (L1 +
     MSE of the normalised distribution: bounded gradients, and L1 is the
     quantity the accuracy gate measures)."""
@@ -171,6 +174,13 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         model = SelectorNet(weighted=name == "GradientSelectorWeight").to(dev)
     else:
         model = FaceNet(plus=name == "Gradient2Predictor").to(dev)
+    if init:
+        st = {k: torch.from_numpy(v) for k, v in W.read_drwnet(init).items()}
+        missing, unexpected = model.load_state_dict(st, strict=False)
+        if unexpected or any(not k.endswith("num_batches_tracked") for k in missing):
+            raise ValueError(f"{init} does not match {name}: missing {missing}, unexpected {unexpected}")
+        model.to(dev)
+        log(f"{name}: starting from {init}")
     faces = {"PoissonPredictor": [0, 1, 2, 3, 4, 5], "Gradient1Predictor": [0, 1],
              "Gradient2Predictor": [2, 3, 4, 5]}.get(name, [None])
     opt = torch.optim.AdamW(model.parameters(), lr=lr, weight_decay=1e-10)
@@ -291,6 +301,9 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
             if e < best:
                 src = avg.module if avg is not None else model
                 best, best_state = e, {k: v.detach().clone() for k, v in src.state_dict().items()}
+                if best_path:
+                    W.write_drwnet(best_path, {k: v.float().cpu().numpy() for k, v in best_state.items()
+                                               if not k.endswith("num_batches_tracked")})
             log(f"{name} step {st + 1}/{total} loss {tot_l / max(nb, 1):.5f} held-out L1 {e:.4f} "
                 f"(best {best:.4f}, {time.time() - t0:.0f}s)")
             tot_l, nb = 0.0, 0
@@ -338,9 +351,9 @@ SHARE = {"PoissonSelector": 0.12, "PoissonPredictor": 0.38, "GradientSelectorWei
          "Gradient2Predictor": 0.24, "Gradient1Predictor": 0.16}
 
 
-def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log, loss_kind="kl", ema=0.999):
+def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log, loss_kind="kl", ema=0.999, init=None):
     m = train_net(name, np.ascontiguousarray(cubes), np.ascontiguousarray(G), np.ascontiguousarray(Gz), dev="cuda",
-                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes, loss_kind=loss_kind, ema=ema)
+                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes, loss_kind=loss_kind, ema=ema, init=init)
     os.makedirs(out_dir, exist_ok=True)
     W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), export_state(m))
     return m
@@ -401,6 +414,7 @@ if __name__ == "__main__":
     ap.add_argument("--make-data", metavar="PATH", help="only build the FDM dataset and save it under PATH.*.npy")
     ap.add_argument("--data", metavar="PATH", help="use a dataset saved by --make-data")
     ap.add_argument("--only", help="train this network only (needs --data)")
+    ap.add_argument("--init", metavar="DRWNET", help="with --only: start from these weights")
     ap.add_argument("--parallel", action="store_true", help="one process per network (needs --data)")
     args = ap.parse_args()
 
@@ -413,7 +427,7 @@ if __name__ == "__main__":
         _log(f"dataset: {args.n} cubes in {time.time() - t0:.1f}s -> {args.make_data}.*.npy")
     elif args.only:
         train_one(args.out, args.only, *load_dataset(args.data), args.minutes, args.batch, args.lr, args.seed, _log,
-                  loss_kind=args.loss, ema=args.ema)
+                  loss_kind=args.loss, ema=args.ema, init=args.init)
     elif args.parallel:
         _log(f"exported {train_parallel(args.out, args.data, args.minutes, args.batch, args.lr, args.seed, _log)}")
     else:
