@@ -168,30 +168,16 @@ __device__ __forceinline__ float gelu(float x)
     return x >= 0.f ? x - xq : xq;
 }
 
-// GELU by table for layers whose output is stored as fp16: Phi(x) is
-// linear between 513 nodes on [-6, 6] (h = 12/512, interpolation error
-// <= h^2/8 max|Phi''| = 1.7e-5, below the fp16 rounding of the result), read
-// from a 512-entry (Phi(x_i), Phi(x_i+1) - Phi(x_i)) table in LDS; outside
-// [-6, 6] the end entries give x (1 - 1e-9) and x 1e-9.  7 VALU
-// instructions and one LDS read.
+// GELU by table: Phi(x) is linear between 513 nodes on [-6, 6] (h = 12/512,
+// interpolation error <= h^2/8 max|Phi''| = 1.7e-5), read from a 512-entry
+// (Phi(x_i), Phi(x_i+1) - Phi(x_i)) table in LDS; outside [-6, 6] the end
+// entries give x (1 - 1e-9) and x 1e-9.  7 VALU instructions and one LDS
+// read.  The face predictors use it for every layer: on the fp32 tail too,
+// the network's error against the fp32 forward stays within its fp16
+// budget (CPU emulation of the kernel's rounding points).  The host builds
+// the table once per network in double precision (nn_host.cpp).
 constexpr int kGeluTab = 512;
 constexpr float kGeluLo = -6.0f, kGeluInvH = kGeluTab / 12.0f;
-
-__device__ __forceinline__ float phi_accurate(float x)
-{
-    const float q = phi_neg_abs(x);
-    return x >= 0.f ? 1.f - q : q;
-}
-
-// Fills tab[0..512) (256 threads); the caller synchronises.
-__device__ __forceinline__ void gelu_table_build(float2* tab)
-{
-    for (int i = threadIdx.x; i < kGeluTab; i += 256) {
-        const float x0 = kGeluLo + (float)i / kGeluInvH, x1 = kGeluLo + (float)(i + 1) / kGeluInvH;
-        const float p0 = phi_accurate(x0), p1 = phi_accurate(x1);
-        tab[i] = make_float2(p0, p1 - p0);
-    }
-}
 
 __device__ __forceinline__ float gelu_tab(float x, const float2* tab)
 {

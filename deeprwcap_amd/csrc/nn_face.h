@@ -21,6 +21,7 @@ struct FaceNetDev {
     const float* dw_w[8];
     const float* dw_b[8];
     const float* head_w;
+    const float2* gtab;  // GELU table (gelu_tab), copied into LDS by every workgroup
     float head_b;
 };
 

@@ -10,9 +10,10 @@
 // Activations live in LDS as fp16 [pixel][channel] (pixel = 23 h + w, 529
 // pixels padded to 544 = 34 MFMA row tiles) up to the last MFMA layer and as
 // fp32 in the tail layers (<= 8 channels).  1x1 convolutions with >= 8
-// outputs run on v_mfma_f32_16x16x16f16 (fp32 accumulate); depthwise 3x3,
-// the small 1x1 layers, batch-norm (folded into weights/bias on the host) and
-// GELU run on the VALU in fp32.
+// outputs run on MFMA (v_mfma_f32_16x16x32_f16 where a lane holds 8 K slots,
+// else 16x16x16; fp32 accumulate); depthwise 3x3, the small 1x1 layers,
+// batch-norm (folded into weights/bias on the host) and GELU (LDS table) run
+// on the VALU in fp32.
 #include "common.h"
 #include "nn_common.h"
 #include "nn_face.h"
@@ -108,14 +109,6 @@ __device__ __forceinline__ void store_pair(half_t* d, float a, float b)
 }
 __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float2*)d = make_float2(a, b); }
 
-// GELU of a value stored as TO: the LDS table where the result is rounded to
-// fp16 anyway, the exact-form approximation for fp32 outputs.
-template <typename TO>
-__device__ __forceinline__ float gelu_for(float x, const float2* gtab)
-{
-    if constexpr (sizeof(TO) == 2) return gelu_tab(x, gtab);
-    else return gelu(x);
-}
 
 // Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU, one
 // output (pixel, channel pair) per item with all 9 taps read from LDS: every
@@ -158,7 +151,7 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
             a0 += w0[k] * v0[k];
             a1 += w1[k] * v1[k];
         }
-        store_pair(out + p * CSO + 2 * cp, gelu_for<TO>(a0, gtab), gelu_for<TO>(a1, gtab));
+        store_pair(out + p * CSO + 2 * cp, gelu_tab(a0, gtab), gelu_tab(a1, gtab));
     }
 }
 
@@ -203,7 +196,7 @@ __device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float
                     a0 += w0[kx] * up[2 * kx] + w0[3 + kx] * mid[2 * kx] + w0[6 + kx] * dn[2 * kx];
                     a1 += w1[kx] * up[2 * kx + 1] + w1[3 + kx] * mid[2 * kx + 1] + w1[6 + kx] * dn[2 * kx + 1];
                 }
-                store_pair(out + (y * kN + x) * CSO + 2 * cp, gelu_for<TO>(a0, gtab), gelu_for<TO>(a1, gtab));
+                store_pair(out + (y * kN + x) * CSO + 2 * cp, gelu_tab(a0, gtab), gelu_tab(a1, gtab));
 #pragma unroll
                 for (int q = 0; q < 6; q++) { up[q] = mid[q]; mid[q] = dn[q]; }
             }
@@ -224,7 +217,7 @@ __device__ __forceinline__ void dw_conv(const TI* in, TO* out, const float* w, c
 // Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4), fp32
 // activations.
 template <int CIN, int COUT, int CSI, int CSO>
-__device__ void pw_valu(const float* in, float* out, const float* w, const float* bias)
+__device__ void pw_valu(const float* in, float* out, const float* w, const float* bias, const float2* gtab)
 {
     for (int p = threadIdx.x; p < kNN; p += 256) {
         float v[CIN];
@@ -235,7 +228,7 @@ __device__ void pw_valu(const float* in, float* out, const float* w, const float
             float a = bias[o];
 #pragma unroll
             for (int c = 0; c < CIN; c++) a += w[o * CIN + c] * v[c];
-            out[p * CSO + o] = gelu(a);
+            out[p * CSO + o] = gelu_tab(a, gtab);
         }
     }
 }
@@ -380,7 +373,7 @@ struct FaceSmem {
     alignas(16) half_t X[kPix * CS];  // 16-byte aligned: read as half4 / float2
     alignas(16) half_t Y[kPix * CS];
     alignas(16) float zero[4];  // read by out-of-image depthwise taps; zeroed at kernel start
-    alignas(16) float2 gtab[kGeluTab];  // GELU table of the fp16-output layers, built at kernel start
+    alignas(16) float2 gtab[kGeluTab];  // GELU table, copied from the network's device copy at kernel start
     VoxDesc desc;
     float red[8];
     int ibuf[4];
@@ -399,9 +392,9 @@ __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
     dw_conv<8, 2, 16, 8>(s.X, fY, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
-    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[3], w.pw_bias[3]); __syncthreads();
+    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[3], w.pw_bias[3], s.gtab); __syncthreads();
     dw_conv<4, 3, 4, 4>(fX, fY, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
-    pw_valu<4, 2, 4, 2>(fY, fX, w.pw_w[4], w.pw_bias[4]); __syncthreads();
+    pw_valu<4, 2, 4, 2>(fY, fX, w.pw_w[4], w.pw_bias[4], s.gtab); __syncthreads();
     head_conv<2, 2>(fX, fY, w.head_w, w.head_b);
 }
 
@@ -423,7 +416,7 @@ __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetD
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
     dw_conv<8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
-    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7]); __syncthreads();
+    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7], s.gtab); __syncthreads();
     head_conv<4, 4>(fX, fY, w.head_w, w.head_b);
 }
 
@@ -453,7 +446,7 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     if (MODE == 2 && !(face >= 2 && face <= 5)) return;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
     if (threadIdx.x < 4) s.zero[threadIdx.x] = 0.f;
-    gelu_table_build(s.gtab);
+    for (int i = threadIdx.x; i < kGeluTab; i += 256) s.gtab[i] = w.gtab[i];
     __syncthreads();
     if constexpr (MODE == 2) face_solver_plus(s, w, face, axis); else face_solver<MODE>(s, w, face, axis);
     __syncthreads();

@@ -153,6 +153,20 @@ std::vector<half_t> pack_frag_input(const std::vector<float>& W, int N)
     return out;
 }
 
+// Table of gelu_tab: (Phi(x_i), Phi(x_i+1) - Phi(x_i)) at the nodes
+// x_i = -6 + i h of [-6, 6], Phi(x) = erfc(-x / sqrt 2) / 2 in double.
+std::vector<float2> gelu_table()
+{
+    std::vector<float2> t(kGeluTab);
+    auto phi = [](double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); };
+    for (int i = 0; i < kGeluTab; i++) {
+        const double x0 = kGeluLo + (double)i / kGeluInvH, x1 = kGeluLo + (double)(i + 1) / kGeluInvH;
+        const double p0 = phi(x0);
+        t[i] = make_float2((float)p0, (float)(phi(x1) - p0));
+    }
+    return t;
+}
+
 std::vector<float> pad(const std::vector<float>& v, size_t n)
 {
     std::vector<float> o(std::max(n, v.size()), 0.f);
@@ -208,6 +222,7 @@ int build_face(const TensorMap& m, const FaceArch& A, Arena& ar, FaceNetDev& d)
     }
     if ((rc = get(m, pre + "head.weight", (size_t)A.clast, &t))) return rc;
     ar.add(pad(t->data, 4), &d.head_w);
+    ar.add(gelu_table(), &d.gtab);
     if ((rc = get(m, pre + "head.bias", 1, &t))) return rc;
     d.head_b = t->data[0];
     return 0;
