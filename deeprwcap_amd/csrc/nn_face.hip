@@ -87,7 +87,6 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
 // keep their activations in fp32: with so few channels per output, fp16
 // rounding of their inputs would dominate the kernel's error budget, and
 // their storage is small.
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void load_pair(const half_t* s, float& a, float& b)
 {
     const half2_t c = *(const half2_t*)s;

@@ -228,6 +228,29 @@ int build_face(const TensorMap& m, const FaceArch& A, Arena& ar, FaceNetDev& d)
     return 0;
 }
 
+// Weights of the selectors' direct first conv (conv1_direct): per output
+// channel 14 fp16 pairs.  Pair r < 9 is tap row (tz, ty) = (r/3, r%3): x-taps
+// (0, 1) when tz + ty is even, else (1, 2); pairs 9..13 hold the remaining
+// taps of rows 0..8 two at a time (x-tap 2 or 0 of each row, in row order;
+// the last pair's second weight is zero).  Tap q = 9 tz + 3 ty + tx of the
+// [C][1][3][3][3] weight.
+std::vector<half_t> conv1_pairs(const std::vector<float>& W, int C)
+{
+    std::vector<half_t> out((size_t)C * 28, (half_t)0.f);
+    for (int c = 0; c < C; c++) {
+        half_t* o = out.data() + (size_t)c * 28;
+        int single[10];
+        for (int r = 0; r < 9; r++) {
+            const int tz = r / 3, ty = r % 3, par = (tz + ty) & 1, q0 = 9 * tz + 3 * ty;
+            o[2 * r] = (half_t)W[(size_t)c * 27 + q0 + par];
+            o[2 * r + 1] = (half_t)W[(size_t)c * 27 + q0 + par + 1];
+            single[r] = q0 + (par ? 0 : 2);
+        }
+        for (int k = 0; k < 9; k++) o[18 + k] = (half_t)W[(size_t)c * 27 + single[k]];
+    }
+    return out;
+}
+
 // FaceSelector / FaceSelectorWeight (architecture.py:176-240) -> SelNetDev
 int build_sel(const TensorMap& m, bool weighted, Arena& ar, SelNetDev& d)
 {
@@ -249,6 +272,7 @@ int build_sel(const TensorMap& m, bool weighted, Arena& ar, SelNetDev& d)
         }
         ar.add(pack_frag(W, co, K), (const half_t**)&d.frag[l]);
         ar.add(pad(b, 64), &d.bias[l]);
+        if (l == 0) ar.add(conv1_pairs(W, co), &d.conv1_w);
     }
     const std::vector<int> dims = weighted ? std::vector<int>{512, 256, 128, 7} : std::vector<int>{128, 64, 6};
     for (size_t j = 0; j + 1 < dims.size(); j++) {

@@ -12,6 +12,7 @@ namespace drw {
 struct SelNetDev {
     const half4_t* frag[6];
     const float* bias[6];
+    const half_t* conv1_w;  // first Conv3d for the direct form: per channel 14 fp16 tap pairs (conv1_pairs)
 };
 
 struct SelBackboneArgs {

@@ -4,7 +4,9 @@
 // selector_backbone_kernel: one 256-thread workgroup per query voxelizes the
 //   (axis pre-rotated) cube into LDS as fp16 and runs the three
 //   Conv3d(k3, s2) + folded BN + GELU layers as implicit GEMMs on
-//   v_mfma_f32_16x16x16f16 (rows = output positions, k = ci*27 + tap).
+//   v_mfma_f32_16x16x16f16 (rows = output positions, k = ci*27 + tap),
+//   except FaceSelector's first conv (1 -> 4 channels), which runs on the
+//   VALU (conv1_direct).
 //   Writes the flattened features [B][C3 * 8].
 // selector_head_kernel: 16 queries per workgroup, the Linear layers as MFMA
 //   GEMMs, softmax over the six face logits, the face draw with the query's
@@ -150,6 +152,80 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
     if constexpr (SLABS) __syncthreads();  // outputs visible; koff free for the next layer
 }
 
+// First Conv3d (1 -> C1, k3, s2, no padding) + folded BN + GELU on the VALU,
+// one output z-slab per phase with the same in-place layout as
+// conv3d_mfma<..., SLABS>: out[oz][c][oy*11+ox] over the cube planes that
+// slab oz no longer needs.  An item is one output pixel of the slab and one
+// channel pair; its 27 taps lie inside the cube for every output, so they are
+// LDS reads at fixed offsets from one base address.  The slab base offset is
+// even and the plane and row strides (529, 23) are odd, so in tap row
+// (tz, ty) the x-taps 0, 1 (tz + ty even) or 1, 2 (odd) form a 4-byte-aligned
+// pair: a row is one 32-bit and one 16-bit read, and the products are
+// v_dot2_f32_f16 (fp16 inputs and weights, fp32 accumulate) over the nine
+// pairs and the nine remaining taps paired up in row order (conv1_pairs on
+// the host lays the weights out the same way).  With one input channel
+// (K = 27) and 4 outputs, the implicit-GEMM form spends most of its
+// instructions on the gather and on moving the 4 valid columns out of the
+// 16-wide MFMA tile.
+template <int C1>
+__device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, const float* bias)
+{
+    constexpr int DO = 11, DO2 = DO * DO, CP = C1 / 2, NP = 14;
+    constexpr int IPT = (DO2 * CP + 255) / 256;  // items per thread and phase
+    static_assert(C1 % 2 == 0 && 256 % CP == 0, "a thread's channel pair stays fixed");
+    static_assert(C1 * DO2 <= 2 * kNN, "slab outputs must fit the freed planes");
+    const int cp = threadIdx.x % CP;
+    half2_t w0[NP], w1[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+        w0[k] = ((const half2_t*)w)[(2 * cp) * NP + k];
+        w1[k] = ((const half2_t*)w)[(2 * cp + 1) * NP + k];
+    }
+    const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
+    for (int oz = 0; oz < DO; oz++) {
+        float held[IPT][2];
+        int pix[IPT];
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            const int mm = (threadIdx.x + 256 * u) / CP;
+            pix[u] = mm;
+            float a0 = b0, a1 = b1;
+            if (mm < DO2) {
+                const int oy = mm / DO, ox = mm - oy * DO;
+                const half_t* src = cube + 2 * oz * kNN + 2 * oy * kN + 2 * ox;
+                half_t single[9];
+#pragma unroll
+                for (int r = 0; r < 9; r++) {
+                    const int tz = r / 3, ty = r % 3, par = (tz + ty) & 1;
+                    const half_t* row = src + tz * kNN + ty * kN;
+                    const half2_t pr = *(const half2_t*)(row + par);
+                    single[r] = row[par ? 0 : 2];
+                    a0 = __builtin_amdgcn_fdot2(pr, w0[r], a0, false);
+                    a1 = __builtin_amdgcn_fdot2(pr, w1[r], a1, false);
+                }
+#pragma unroll
+                for (int k = 0; k < 5; k++) {
+                    const half2_t sp = {single[2 * k], k < 4 ? single[2 * k + 1] : (half_t)0.f};
+                    a0 = __builtin_amdgcn_fdot2(sp, w0[9 + k], a0, false);
+                    a1 = __builtin_amdgcn_fdot2(sp, w1[9 + k], a1, false);
+                }
+            }
+            held[u][0] = gelu(a0);
+            held[u][1] = gelu(a1);
+        }
+        __syncthreads();  // this slab's input planes fully read
+        half_t* o = out + oz * (C1 * DO2);
+#pragma unroll
+        for (int u = 0; u < IPT; u++) {
+            if (pix[u] < DO2) {
+                o[(2 * cp) * DO2 + pix[u]] = (half_t)held[u][0];
+                o[(2 * cp + 1) * DO2 + pix[u]] = (half_t)held[u][1];
+            }
+        }
+    }
+    __syncthreads();  // outputs visible
+}
+
 // LDS of one query: the cube, the first conv's output (written slab by slab
 // over the cube planes it no longer needs, slab-major [oz][c][11*11]) and
 // the second conv's output ([c][125], after the first's) share one region:
@@ -208,7 +284,8 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
     __syncthreads();
     half_t* l1 = s.cube;                 // [11][C1][121], over the cube
     half_t* l2 = s.cube + C1 * 1331;     // [C2][125], after l1
-    conv3d_mfma<1, C1, 23, 11, 0, kNN, kN, true, false>(s.cube, l1, w.frag[0], w.bias[0], s.koff);
+    if constexpr (C1 == 4) conv1_direct<C1>(s.cube, l1, w.conv1_w, w.bias[0]);
+    else conv3d_mfma<1, C1, 23, 11, 0, kNN, kN, true, false>(s.cube, l1, w.frag[0], w.bias[0], s.koff);
     conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false, false>(l1, l2, w.frag[1], w.bias[1], s.koff);
     __syncthreads();
     conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, false, true>(l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2],
