@@ -5,8 +5,8 @@
 //   (axis pre-rotated) cube into LDS as fp16 and runs the three
 //   Conv3d(k3, s2) + folded BN + GELU layers as implicit GEMMs on
 //   v_mfma_f32_16x16x16f16 (rows = output positions, k = ci*27 + tap),
-//   except FaceSelector's first conv (1 -> 4 channels), which runs on the
-//   VALU (conv1_direct).
+//   except the first conv (one input channel), which runs on the VALU
+//   (conv1_direct).
 //   Writes the flattened features [B][C3 * 8].
 // selector_head_kernel: 16 queries per workgroup, the Linear layers as MFMA
 //   GEMMs, softmax over the six face logits, the face draw with the query's
@@ -21,31 +21,22 @@ namespace drw {
 
 // Conv3d(CIN -> COUT, k3, s2, no padding) over in[ci][z][y][x] (fp16,
 // strides SC, SZ, SY, 1) as an implicit GEMM (rows = output positions,
-// k = ci*27 + tap) with bias + GELU.  koff (LDS, >= KT*16 ints) receives the
-// im2col offset of every k (-1 past K), so a gathered element costs one add
-// and one LDS read.  With COUT < 16 only COUT of the 16 MFMA columns hold
-// outputs; the valid ones are first moved across lanes (ds_bpermute) so that
-// every lane evaluates 16*COUT/64 GELUs instead of 4.
-//
-// Output layouts:
-// * SLABS: fp16 out[oz][co][oy*DOUT+ox], one output z-slab per phase with a
-//   barrier after it.  Slab oz reads input planes 2oz..2oz+2, so after its
-//   barrier the planes below 2oz+2 are dead and the slab's outputs (held in
-//   registers until then) are written over them: `out` may alias `in`
-//   (requires COUT * DOUT^2 <= 2 SZ).
-// * otherwise out[co][oz*DOUT^2+oy*DOUT+ox], fp16 or fp32 (FP32_OUT); `out`
-//   must not overlap `in`.
-template <int CIN, int COUT, int DIN, int DOUT, int SC, int SZ, int SY, bool SLABS, bool FP32_OUT>
+// k = ci*27 + tap) with bias + GELU, into out[co][oz*DOUT^2+oy*DOUT+ox],
+// fp16 or fp32 (FP32_OUT); `out` must not overlap `in`.  koff (LDS,
+// >= KT*16 ints) receives the im2col offset of every k (-1 past K), so a
+// gathered element costs one add and one LDS read.  With COUT < 16 only COUT
+// of the 16 MFMA columns hold outputs; the valid ones are first moved across
+// lanes (ds_bpermute) so that every lane evaluates 16*COUT/64 GELUs instead
+// of 4.  Ends with a barrier: outputs visible, koff free for the next layer.
+template <int CIN, int COUT, int DIN, int DOUT, int SC, int SZ, int SY, bool FP32_OUT>
 __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, const float* bias, int* koff)
 {
     constexpr int DO2 = DOUT * DOUT, M = DOUT * DO2;
     constexpr int K = CIN * 27, KT = (K + 15) / 16;
     constexpr int NT = (COUT + 15) / 16;
     constexpr int VPT = COUT < 16 ? 16 * COUT / 64 : 4 * NT;  // outputs per lane per tile
-    constexpr int MP = SLABS ? DO2 : M;                      // positions per phase
-    constexpr int TP = (MP + 15) / 16, TPW = (TP + 3) / 4;   // tiles per phase / per wave
+    constexpr int TP = (M + 15) / 16, TPW = (TP + 3) / 4;    // tiles / tiles per wave
     static_assert(COUT >= 16 || (16 * COUT) % 64 == 0, "compacted tiles need COUT a multiple of 4");
-    static_assert(!SLABS || (COUT * DO2 <= 2 * SZ && !FP32_OUT), "slab outputs must fit the freed planes");
     for (int k = threadIdx.x; k < KT * 16; k += 256) {
         int o = -1;
         if (k < K) {
@@ -64,109 +55,81 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 #pragma unroll
             for (int kt = 0; kt < KT; kt++) bw[n][kt] = frag[(n * KT + kt) * 64 + lane];
     }
-    // (channel, position within the phase) of value j of tile t held by this lane
-    auto where = [&](int t, int j, int& ch, int& mm) {
-        if constexpr (COUT < 16) {
-            const int o = lane * VPT + j;
-            ch = o % COUT;
-            mm = t * 16 + o / COUT;
-        } else {
-            ch = (j >> 2) * 16 + r;
-            mm = t * 16 + kq + (j & 3);
+#pragma unroll
+    for (int u = 0; u < TPW; u++) {
+        const int t = wave + 4 * u;
+        if (t >= TP) continue;
+        const int m = t * 16 + r;
+        int base = -1;
+        if (m < M) {
+            const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
+            base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
         }
-    };
-    constexpr int PHASES = SLABS ? DOUT : 1;
-    for (int ph = 0; ph < PHASES; ph++) {
-        float held[TPW * VPT];
+        float4_t acc[NT];
 #pragma unroll
-        for (int u = 0; u < TPW; u++) {
-            const int t = wave + 4 * u;
-            if (t >= TP) continue;
-            const int mloc = t * 16 + r, m = ph * MP + mloc;
-            int base = -1;
-            if (mloc < MP) {
-                const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
-                base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
-            }
-            float4_t acc[NT];
+        for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
+        for (int kt = 0; kt < KT; kt++) {
+            const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
+            const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
+            half4_t a;
 #pragma unroll
-            for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
-            for (int kt = 0; kt < KT; kt++) {
-                const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
-                const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
-                half4_t a;
+            for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
 #pragma unroll
-                for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
-#pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    half4_t b;
-                    if constexpr (KT <= 2) b = bw[n][kt];
-                    else b = frag[(n * KT + kt) * 64 + lane];
-                    acc[n] = mfma16(a, b, acc[n]);
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < VPT; j++) {
-                float v;
-                int ch, mm;
-                where(t, j, ch, mm);
-                if constexpr (COUT < 16) {
-                    // output o = lane*VPT + j -> (row, col) = (o / COUT, o % COUT),
-                    // held by lane col + 16*(row/4) as element row % 4
-                    const int row = (lane * VPT + j) / COUT;
-                    const int src = ch + 16 * (row >> 2), el = row & 3;
-                    v = 0.f;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const float tv = __shfl(acc[0][i], src, 64);
-                        v = el == i ? tv : v;
-                    }
-                } else {
-                    v = acc[j >> 2][j & 3];
-                }
-                const float g = gelu(v + bias[ch]);
-                if constexpr (SLABS) {
-                    held[u * VPT + j] = g;
-                } else if (mm < MP) {
-                    if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
-                    else ((half_t*)outp)[ch * M + mm] = (half_t)g;
-                }
+            for (int n = 0; n < NT; n++) {
+                half4_t b;
+                if constexpr (KT <= 2) b = bw[n][kt];
+                else b = frag[(n * KT + kt) * 64 + lane];
+                acc[n] = mfma16(a, b, acc[n]);
             }
         }
-        __syncthreads();  // this phase's inputs fully read
-        if constexpr (SLABS) {
-            half_t* o = (half_t*)outp + ph * (COUT * DO2);
 #pragma unroll
-            for (int u = 0; u < TPW; u++) {
-                const int t = wave + 4 * u;
-                if (t >= TP) continue;
+        for (int j = 0; j < VPT; j++) {
+            float v;
+            int ch, mm;
+            if constexpr (COUT < 16) {
+                // output o = lane*VPT + j -> (row, col) = (o / COUT, o % COUT),
+                // held by lane col + 16*(row/4) as element row % 4
+                const int o = lane * VPT + j, row = o / COUT;
+                ch = o % COUT;
+                mm = t * 16 + row;
+                const int src = ch + 16 * (row >> 2), el = row & 3;
+                v = 0.f;
 #pragma unroll
-                for (int j = 0; j < VPT; j++) {
-                    int ch, mm;
-                    where(t, j, ch, mm);
-                    if (mm < MP) o[ch * DO2 + mm] = (half_t)held[u * VPT + j];
+                for (int i = 0; i < 4; i++) {
+                    const float tv = __shfl(acc[0][i], src, 64);
+                    v = el == i ? tv : v;
                 }
+            } else {
+                ch = (j >> 2) * 16 + r;
+                mm = t * 16 + kq + (j & 3);
+                v = acc[j >> 2][j & 3];
+            }
+            const float g = gelu(v + bias[ch]);
+            if (mm < M) {
+                if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
+                else ((half_t*)outp)[ch * M + mm] = (half_t)g;
             }
         }
     }
-    if constexpr (SLABS) __syncthreads();  // outputs visible; koff free for the next layer
+    __syncthreads();
 }
 
 // First Conv3d (1 -> C1, k3, s2, no padding) + folded BN + GELU on the VALU,
-// one output z-slab per phase with the same in-place layout as
-// conv3d_mfma<..., SLABS>: out[oz][c][oy*11+ox] over the cube planes that
-// slab oz no longer needs.  An item is one output pixel of the slab and one
-// channel pair; its 27 taps lie inside the cube for every output, so they are
-// LDS reads at fixed offsets from one base address.  The slab base offset is
-// even and the plane and row strides (529, 23) are odd, so in tap row
-// (tz, ty) the x-taps 0, 1 (tz + ty even) or 1, 2 (odd) form a 4-byte-aligned
-// pair: a row is one 32-bit and one 16-bit read, and the products are
-// v_dot2_f32_f16 (fp16 inputs and weights, fp32 accumulate) over the nine
-// pairs and the nine remaining taps paired up in row order (conv1_pairs on
-// the host lays the weights out the same way).  With one input channel
-// (K = 27) and 4 outputs, the implicit-GEMM form spends most of its
-// instructions on the gather and on moving the 4 valid columns out of the
-// 16-wide MFMA tile.
+// one output z-slab per phase, in place: slab oz reads input planes
+// 2oz..2oz+2, so after the phase's barrier the planes below 2oz+2 are dead
+// and the slab's outputs (held in registers until then) are written over
+// them as out[oz][c][oy*11+ox] (C1 * 121 <= 2 * 529).  An item is one output
+// pixel of the slab and one channel pair; its 27 taps lie inside the cube
+// for every output, so they are LDS reads at fixed offsets from one base
+// address.  The slab base offset is even and the plane and row strides
+// (529, 23) are odd, so in tap row (tz, ty) the x-taps 0, 1 (tz + ty even)
+// or 1, 2 (odd) form a 4-byte-aligned pair: a row is one 32-bit and one
+// 16-bit read, and the products are v_dot2_f32_f16 (fp16 inputs and
+// weights, fp32 accumulate) over the nine pairs and the nine remaining taps
+// paired up in row order (conv1_pairs on the host lays the weights out the
+// same way).  With one input channel (K = 27) and 4 or 8 outputs, the
+// implicit-GEMM form spends most of its instructions on the gather and on
+// moving the valid columns out of the 16-wide MFMA tile.
 template <int C1>
 __device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, const float* bias)
 {
@@ -284,12 +247,9 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
     __syncthreads();
     half_t* l1 = s.cube;                 // [11][C1][121], over the cube
     half_t* l2 = s.cube + C1 * 1331;     // [C2][125], after l1
-    if constexpr (C1 == 4) conv1_direct<C1>(s.cube, l1, w.conv1_w, w.bias[0]);
-    else conv3d_mfma<1, C1, 23, 11, 0, kNN, kN, true, false>(s.cube, l1, w.frag[0], w.bias[0], s.koff);
-    conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false, false>(l1, l2, w.frag[1], w.bias[1], s.koff);
-    __syncthreads();
-    conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, false, true>(l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2],
-                                                       s.koff);
+    conv1_direct<C1>(s.cube, l1, w.conv1_w, w.bias[0]);
+    conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false>(l1, l2, w.frag[1], w.bias[1], s.koff);
+    conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, true>(l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2], s.koff);
 }
 
 // Linear layer on MFMA for 16 rows: X[16][K] (fp16, LDS) -> Y[16][N].
