@@ -102,6 +102,64 @@ def fdm_roofline(L, C, kernel_ms):
                     "barrier latency (one 1024-thread workgroup per CU)"}
 
 
+def fdm_batch(torch, B=4096, seed=0):
+    """Config 2: B cubes with N uniform over {16, 21, 23} (1-3 planar layers
+    + 0-2 boxes, eps U[1,8]) solved in one FDM launch at tol 1e-6 (GFT);
+    cubes/s, mean CG iterations and K*25*N^3 FLOP against the fp32 vector
+    peak."""
+    from deeprwcap_amd import device as D, structures as S
+    Ns = np.random.default_rng(seed).choice(np.array([16, 21, 23], np.int32), size=B)
+    L, C, _ = S.layered_boxes(B, Ns, seed=seed)
+    pk = D.PackedCubes(Ns)
+    D.voxelize(L, C, N=Ns, normalise=False, out=pk.eps, out_off=pk.d_eps_off)
+    D.fdm_solve(pk, 0, tol=1e-6)
+    torch.cuda.synchronize()
+    reps, dts = 3, []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        iters, _ = D.fdm_solve(pk, 0, tol=1e-6)
+        torch.cuda.synchronize()
+        dts.append(time.perf_counter() - t0)
+    dt = float(np.median(dts))
+    K = iters.cpu().numpy().astype(np.float64)
+    flop = 25.0 * float((K * Ns.astype(np.float64) ** 3).sum())
+    achieved = flop / dt / 1e12
+    return {"cubes": B, "N": "uniform {16, 21, 23}", "seconds": dt, "cubes_per_s": B / dt,
+            "iters_mean": float(K.mean()), "achieved_tflops": achieved,
+            "frac_fp32_vector_peak": achieved / FP32_VALU_PEAK_TFLOPS}
+
+
+def fdm_cpu_reference(L, C, n=400):
+    """The reference GGFT solver (BiCGSTAB, vendored Eigen 3.4), built from
+    its own sources into oracle/_ref, on the host cores: one process per
+    core, as its dataset driver runs one solver per thread
+    (ggft/src/main.cpp:38-57), each solving the first n cubes of the
+    benchmark batch (N = 23, GFT, tol 1e-6).  Solves per second."""
+    import struct
+    import subprocess
+    from concurrent.futures import ThreadPoolExecutor
+    exe = os.path.join(ROOT, "oracle", "_ref", "ggft_ref")
+    if not os.path.exists(exe):
+        return {"value": None, "note": "oracle/_ref/ggft_ref not built (needs the reference sources at build time)"}
+    from oracle import oracle as O
+    cubes, _ = O.voxelize(L[:n], C[:n], 23, normalise=False)
+    inp = struct.pack("<iid", n, 0, 1e-6) + np.ascontiguousarray(cubes, np.float64).tobytes()
+    P = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    procs = [subprocess.Popen([exe], stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL)
+             for _ in range(P)]
+    with ThreadPoolExecutor(P) as ex:
+        outs = list(ex.map(lambda pr: pr.communicate(inp)[0], procs))
+    wall = time.perf_counter() - t0
+    if any(pr.returncode != 0 for pr in procs) or any(len(o) != n * (6 * 529 + 1) * 8 for o in outs):
+        return {"value": None, "note": "reference GGFT run failed"}
+    solve_s = float(np.mean([np.frombuffer(o, np.float64).reshape(n, -1)[:, -1].mean() for o in outs]))
+    return {"value": P * n / wall, "unit": "FDM solves/s (N=23, GFT, tol 1e-6)", "cores": P, "kind": "reference",
+            "seconds_per_solve": solve_s,
+            "sample": f"{P} processes x {n} cubes of the benchmark batch through the reference GGFT solver "
+                      f"(oracle/_ref, BiCGSTAB + Eigen 3.4), {wall:.1f}s"}
+
+
 def l1_vs_fdm(solver, L, C, n, torch):
     """Full-kernel L1 of the networks against FP64 FDM kernels (N = 23)."""
     from deeprwcap_amd import _lib, device as D
@@ -331,6 +389,9 @@ def main():
     log(f"FDM: {fdm_rate:,.0f} queries/s (FDM kernel {fdm_kernel_ms:.3f} ms per launch)")
     fdm_roof = fdm_roofline(L, C, fdm_kernel_ms)
     log(f"FDM kernel: {fdm_roof['iters_mean']:.1f} CG iterations per cube, {fdm_roof['achieved']:.1f} TFLOP/s")
+    fdm_c2 = fdm_batch(torch)
+    log(f"FDM batch (4096 cubes, N in {{16,21,23}}): {fdm_c2['cubes_per_s']:,.0f} cubes/s, "
+        f"{fdm_c2['iters_mean']:.1f} iterations, {fdm_c2['achieved_tflops']:.1f} TFLOP/s")
 
     # accuracy gate: network full kernels vs FP64 FDM
     l1_mean, l1_p99 = l1_vs_fdm(nn, L, C, 256, torch)
@@ -366,6 +427,7 @@ def main():
         "fdm_queries_per_s": fdm_rate,
         "nn_over_fdm": value / fdm_rate,
         "fdm_roofline": fdm_roof,
+        "fdm_batch": fdm_c2,
         "l1_vs_fp64_mean": l1_mean,
         "l1_vs_fp64_p99": l1_p99,
         "latency_ms_batch64": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
@@ -385,6 +447,8 @@ def main():
     if not args.no_cpu:
         log("timing the CPU path ...")
         res["cpu_baseline"] = cpu_baseline(L, C, models_txt)
+        log("timing the reference GGFT solver on the host ...")
+        res["fdm_cpu_baseline"] = fdm_cpu_reference(L, C)
     print(json.dumps(res), flush=True)
 
 
