@@ -32,6 +32,10 @@ SEL_FLOP = 2 * 287_972       # PoissonSelector MACs per query (architecture.py:1
 HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TFLOPS = 2500.0
 FP32_VALU_PEAK_TFLOPS = 157.3
+# Structures of the accuracy gate: a seed no training set uses (the training
+# datasets draw from seeds above 10^6, deeprwcap_amd/training.py).
+EVAL_SEED = 0x5EED0000
+EVAL_QUERIES = 512
 
 
 def log(msg):
@@ -394,7 +398,8 @@ def main():
         f"{fdm_c2['iters_mean']:.1f} iterations, {fdm_c2['achieved_tflops']:.1f} TFLOP/s")
 
     # accuracy gate: network full kernels vs FP64 FDM
-    l1_mean, l1_p99 = l1_vs_fdm(nn, L, C, 256, torch)
+    Le, Ce, _ = S.layered_boxes(EVAL_QUERIES, 23, seed=EVAL_SEED)
+    l1_mean, l1_p99 = l1_vs_fdm(nn, Le, Ce, EVAL_QUERIES, torch)
     log(f"L1 vs FP64 FDM: mean {l1_mean:.4f} p99 {l1_p99:.4f}")
 
     # small-batch latency (one synchronous batch of 64 queries)

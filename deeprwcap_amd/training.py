@@ -115,7 +115,8 @@ def fdm_dataset(n: int, seed: int, tol: float = 1e-10, max_boxes: int = 2, chunk
     Gz = np.empty((n, 6, 529), np.float32)
     for c0 in range(0, n, chunk):
         b = min(chunk, n - c0)
-        L, C, _ = S.layered_boxes(b, NET_N, seed=seed * 100003 + c0, max_boxes=max_boxes)
+        # seeds above 10^6: disjoint from the benchmark's and the tests' structures
+        L, C, _ = S.layered_boxes(b, NET_N, seed=1_000_003 * (seed + 1) + c0, max_boxes=max_boxes)
         pk = D.PackedCubes([NET_N] * b, device=dev)
         D.voxelize(L, C, fixed_N=NET_N, normalise=True, out=pk.eps.view(b, NET_N, NET_N, NET_N))
         cubes[c0:c0 + b] = pk.eps.view(b, NET_N, NET_N, NET_N).cpu().numpy().astype(np.float16)
