@@ -316,7 +316,9 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         if dev != "cpu":
             torch.cuda.synchronize()
         per = (time.time() - t1) / 30
-        total = max(100, int((minutes * 60 - (time.time() - t0)) / (per * (1 + 2.0 / max(eval_every, 1)))))
+        # 25 % margin: the step time measured here underestimates a GPU shared
+        # with other trainings
+        total = max(100, int((minutes * 60 - (time.time() - t0)) / (1.25 * per * (1 + 2.0 / max(eval_every, 1)))))
         log(f"{name}: {per * 1e3:.1f} ms/step at batch {batch} -> {total} steps")
     else:
         total = epochs * max(1, (ntr * len(faces)) // batch)
@@ -383,9 +385,10 @@ SHARE = {"PoissonSelector": 0.12, "PoissonPredictor": 0.38, "GradientSelectorWei
 
 
 def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log, loss_kind="kl", ema=0.999, init=None):
-    m = train_net(name, np.ascontiguousarray(cubes), np.ascontiguousarray(G), np.ascontiguousarray(Gz), dev="cuda",
-                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes, loss_kind=loss_kind, ema=ema, init=init)
     os.makedirs(out_dir, exist_ok=True)
+    m = train_net(name, np.ascontiguousarray(cubes), np.ascontiguousarray(G), np.ascontiguousarray(Gz), dev="cuda",
+                  log=log, batch=batch, lr=lr, seed=seed, minutes=minutes, loss_kind=loss_kind, ema=ema, init=init,
+                  best_path=os.path.join(out_dir, f"{name}.drwnet"))
     W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), export_state(m))
     return m
 
