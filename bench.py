@@ -327,7 +327,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the query path has no cross-GPU exchange; see DESIGN.md for how it shards)", file=sys.stderr)
+              "(the query path has no cross-GPU exchange: walker threads shard their queries over the GPUs, "
+              "each holding its own copy of the weights)", file=sys.stderr)
         sys.exit(2)
 
     import torch

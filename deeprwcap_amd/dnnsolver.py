@@ -53,7 +53,7 @@ class DNNSolver:
             print(f"\tModels loaded from {models_txt}")
 
     def use_fdm(self, n: int = 23, tol: float = 1e-6):
-        """Route queries through the batched FDM solve (fallback / judge)."""
+        """Route queries through the batched FDM solve (the fallback and accuracy reference)."""
         check(_lib.lib().drw_solver_use_fdm(self._h, int(n), float(tol)), "use_fdm")
 
     def set_seed(self, seed: int, counter: int = 0):

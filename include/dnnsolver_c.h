@@ -46,7 +46,7 @@ const char* drw_last_error(void);
 int drw_abi_version(void);
 
 /* ------------------------------------------------------------------------
- * Batched FDM surface Green's function (the GGFT judge / fallback).
+ * Batched FDM surface Green's function (the GGFT accuracy reference / fallback).
  * Replaces ggft_solver::solve (ggft/src/ggft_solver.cpp:521-636) for a batch:
  * per cube b of resolution N[b] in {8, 9, 16, 21, 23}, right-hand side
  * code[b] (DRW_GFT/WVTZ/WVTX/WVTY), writes G = 2 eps y on the 6 N^2 panels
@@ -91,7 +91,7 @@ void drw_solver_destroy(drw_solver* s);
  * selects the network backend. */
 int drw_solver_load_models(drw_solver* s, const char* models_txt);
 
-/* Select the FDM backend (the plain batched-FDM fallback and judge): queries
+/* Select the FDM backend (the plain batched-FDM fallback and accuracy reference): queries
  * are voxelized at resolution N and solved to relative residual tol. */
 int drw_solver_use_fdm(drw_solver* s, int N, double tol);
 int drw_solver_backend(const drw_solver* s);
@@ -155,7 +155,7 @@ int drw_solver_sample_ex(drw_solver* s, int B, const float* layers, const float*
                          void* stream);
 
 /* Full transition kernels from the networks (no draws), for accuracy checks
- * against the FDM judge: probs_out [B][6], kern_out [B][6][529] (face f of
+ * against the FDM reference: probs_out [B][6], kern_out [B][6][529] (face f of
  * the query's frame, tile t = 23 b + a in GGFT panel order), weight_out [B]
  * (gradient; required there).  Poisson: G[f][t] = probs[f] * kern[f][t];
  * gradient: G[remap(f)][t] = W * probs[f] * kern[f][t]. */
