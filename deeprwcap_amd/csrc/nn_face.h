@@ -11,11 +11,11 @@ namespace drw {
 // the pointwise conv of block i.  MFMA layers use frag[l] (B fragments,
 // [n_tile][k_tile][64 lanes] of half4), the small VALU layers pw_w[l]
 // ([cout][cin] fp32).  dw_w[i] is [cin][9] (ky-major), dw_b[i] [cin].
-// frag0_lo holds the rounding residue W - fp16(W) of layer 0, whose product
-// runs in split precision.
+// frag_lo[l] holds the rounding residue W - fp16(W) of MFMA layer l: the
+// 1x1 products run in split precision, W a ~= W_hi a + W_lo a.
 struct FaceNetDev {
     const half4_t* frag[9];
-    const half4_t* frag0_lo;
+    const half4_t* frag_lo[9];
     const float* pw_w[9];
     const float* pw_bias[9];
     const float* dw_w[8];

@@ -26,19 +26,26 @@ namespace drw {
 // The product is formed transposed, D[channel][pixel] = W x X^T (weights as
 // the A operand, activations as B; the packed fragments are the same
 // registers in either role), so a lane ends with 4 consecutive channels of
-// one pixel and writes them with one 8-byte store.
+// one pixel and writes them with one 8-byte store.  The weights enter in two
+// fp16 parts (frag = fp16(W), frag_lo = W - fp16(W)) accumulated in separate
+// chains: with fp16 weights alone the FaceSolverPlus kernels come within a
+// few percent of the 1e-3 L1 bound against the fp32 forward.
 template <int CIN_PAD, int COUT, int COUT_PAD, int CSI, int CSO>
-__device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const float* bias, const float2* gtab)
+__device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const half4_t* frag_lo, const float* bias,
+                        const float2* gtab)
 {
     constexpr int KT = CIN_PAD / 16, NT = COUT_PAD / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = 4 * (lane >> 4);
-    half4_t bw[NT][KT];
+    half4_t bw[NT][KT], bl[NT][KT];
     float bb[NT][4];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
 #pragma unroll
-        for (int k = 0; k < KT; k++) bw[n][k] = frag[(n * KT + k) * 64 + lane];
+        for (int k = 0; k < KT; k++) {
+            bw[n][k] = frag[(n * KT + k) * 64 + lane];
+            bl[n][k] = frag_lo[(n * KT + k) * 64 + lane];
+        }
         const int V = COUT - n * 16 < 16 ? COUT - n * 16 : 16;
         // full tile: channels kq..kq+3; half tile (see below): two channels
         const int c0 = V == 16 ? kq : ((lane >> 5) ? 2 : 0) + 4 * ((lane >> 4) & 1);
@@ -52,13 +59,19 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
         half_t* orow = out + (m * 16 + r) * CSO;
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            float4_t acc = {0.f, 0.f, 0.f, 0.f};
+            float4_t acc = {0.f, 0.f, 0.f, 0.f}, acl = {0.f, 0.f, 0.f, 0.f};
             if constexpr (KT == 2) {
-                acc = mfma32(cat8(bw[n][0], bw[n][1]), cat8(a[0], a[1]), acc);
+                const half8_t a8 = cat8(a[0], a[1]);
+                acl = mfma32(cat8(bl[n][0], bl[n][1]), a8, acl);
+                acc = mfma32(cat8(bw[n][0], bw[n][1]), a8, acc);
             } else {
 #pragma unroll
-                for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
+                for (int k = 0; k < KT; k++) {
+                    acl = mfma16(bl[n][k], a[k], acl);
+                    acc = mfma16(bw[n][k], a[k], acc);
+                }
             }
+            acc += acl;
             constexpr int VFULL = 16;
             const int V = COUT - n * 16 < VFULL ? COUT - n * 16 : VFULL;  // valid channels of this tile
             if (V == VFULL) {
@@ -382,11 +395,11 @@ struct FaceSmem {
 template <int MODE>
 __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.frag0_lo, w.pw_bias[0], s.gtab); __syncthreads();
+    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.frag_lo[0], w.pw_bias[0], s.gtab); __syncthreads();
     dw_conv<16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.pw_bias[1], s.gtab); __syncthreads();
+    pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
     dw_conv<16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.pw_bias[2], s.gtab); __syncthreads();
+    pw_mfma<16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
     // fp32 tail
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
@@ -399,19 +412,19 @@ __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w
 
 __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.frag0_lo, w.pw_bias[0], s.gtab); __syncthreads();
+    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.frag_lo[0], w.pw_bias[0], s.gtab); __syncthreads();
     dw_conv<32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.pw_bias[1], s.gtab); __syncthreads();
+    pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
     dw_conv<32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.pw_bias[2], s.gtab); __syncthreads();
+    pw_mfma<32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
     dw_conv<24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.pw_bias[3], s.gtab); __syncthreads();
+    pw_mfma<32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.frag_lo[3], w.pw_bias[3], s.gtab); __syncthreads();
     dw_conv<24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.pw_bias[4], s.gtab); __syncthreads();
+    pw_mfma<32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
     dw_conv<16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.pw_bias[5], s.gtab); __syncthreads();
+    pw_mfma<16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
     dw_conv<16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.pw_bias[6], s.gtab); __syncthreads();
+    pw_mfma<16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
     dw_conv<8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();

@@ -200,7 +200,7 @@ int build_face(const TensorMap& m, const FaceArch& A, Arena& ar, FaceNetDev& d)
         std::vector<float> Wlo(W.size());
         for (size_t i = 0; i < W.size(); i++) Wlo[i] = W[i] - (float)(half_t)W[i];
         ar.add(pack_frag_input(W, A.c0), (const half_t**)&d.frag[0]);
-        ar.add(pack_frag_input(Wlo, A.c0), (const half_t**)&d.frag0_lo);
+        ar.add(pack_frag_input(Wlo, A.c0), (const half_t**)&d.frag_lo[0]);
         ar.add(pad(sh, 32), &d.pw_bias[0]);
     }
     for (size_t i = 0; i < A.cin.size(); i++) {
@@ -216,8 +216,14 @@ int build_face(const TensorMap& m, const FaceArch& A, Arena& ar, FaceNetDev& d)
         std::vector<float> W(t->data);
         for (int o = 0; o < co; o++)
             for (int c = 0; c < ci; c++) W[o * ci + c] *= sc[o];
-        if (co >= 8) ar.add(pack_frag(W, co, ci), (const half_t**)&d.frag[i + 1]);
-        else ar.add(W, &d.pw_w[i + 1]);
+        if (co >= 8) {
+            std::vector<float> Wlo(W.size());
+            for (size_t q = 0; q < W.size(); q++) Wlo[q] = W[q] - (float)(half_t)W[q];
+            ar.add(pack_frag(W, co, ci), (const half_t**)&d.frag[i + 1]);
+            ar.add(pack_frag(Wlo, co, ci), (const half_t**)&d.frag_lo[i + 1]);
+        } else {
+            ar.add(W, &d.pw_w[i + 1]);
+        }
         ar.add(pad(sh, 32), &d.pw_bias[i + 1]);
     }
     if ((rc = get(m, pre + "head.weight", (size_t)A.clast, &t))) return rc;
