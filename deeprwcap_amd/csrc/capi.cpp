@@ -95,6 +95,7 @@ int drw_fdm_solve(int B, const int32_t* N, const int32_t* code, const float* eps
     a.y_ws = ws->y_ws;
     a.s_ws = ws->s_ws;
     a.counters = ws->counters;
+    a.n_hint = 0;  // per-cube N from the device array
     return fdm_launch(a, ws->fdm_grid, (hipStream_t)stream);
 }
 
@@ -255,7 +256,7 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         a.B = B; a.N = s->Nq.p; a.code = s->code.p; a.eps = s->eps.p; a.eps_off = s->eps_off.p;
         a.G = s->G.p; a.G_off = s->G_off.p; a.order = nullptr; a.tol = s->fdm_tol; a.inner_floor = 1e-5f;
         a.max_inner = 2000; a.max_rounds = 12; a.iters = nullptr; a.relres = nullptr;
-        a.y_ws = ws->y_ws; a.s_ws = ws->s_ws; a.counters = ws->counters;
+        a.y_ws = ws->y_ws; a.s_ws = ws->s_ws; a.counters = ws->counters; a.n_hint = N;
         if ((rc = tm->begin(4, st)) || (rc = fdm_launch(a, ws->fdm_grid, st)) || (rc = tm->end(st))) return rc;
         if (grad && (rc = center_eps_launch(B, s->Nq.p, ax, s->eps.p, s->eps_off.p, s->cen.p, st))) return rc;
         FlatSampleArgs fa{B, s->Nq.p, grad ? ax : nullptr, s->G.p, s->G_off.p, grad ? s->cen.p : nullptr,

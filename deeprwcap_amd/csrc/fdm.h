@@ -22,7 +22,8 @@ struct FdmBatch {
     float* relres;           // [B] final FP64 relative residual (optional)
     double* y_ws;            // [grid * kMaxCells] workspace
     float* s_ws;             // [grid * kMaxCells] workspace
-    uint32_t* counters;      // [2] zero at rest; the kernel leaves them zero
+    uint32_t* counters;      // [4] zero at rest; the kernels leave them zero
+    int n_hint;              // N of every cube when all are equal (one size class launches), else 0
 };
 
 bool fdm_supported_n(int n);

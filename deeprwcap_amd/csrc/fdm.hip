@@ -35,22 +35,26 @@
 
 namespace drw {
 
-constexpr int kFdmThreads = 1024;
-constexpr int kFdmWaves = kFdmThreads / 64;
-constexpr int kPGuard = kMaxN * kMaxN;
-constexpr int kPSize = kMaxCells + 2 * kPGuard + 1024;  // slack covers the unused tail slots
-
-struct FdmShared {
-    float p[kPSize];
-    float wz[kMaxCells + kPGuard];  // scaled z+ couplings, zero guard plane below
-    float ap[kMaxCells];            // A p of the current iteration (own cells only)
-    float red[2][kFdmWaves];
-    double redd[kFdmWaves];
+// Two size classes.  Cubes up to N = 23 take a 1024-thread workgroup and
+// 156 KB of LDS, one per CU; cubes up to N = 16 take 256 threads and 53 KB,
+// so three share a CU and each hides the others' barrier and LDS latency
+// (one cube per CU leaves most of it waiting at the block reductions).
+template <int T, int MAXN, int PER_CU>
+struct FdmSharedT {
+    static constexpr int kThreads = T, kWaves = T / 64, kMaxN = MAXN, kPerCU = PER_CU;
+    static constexpr int kGuard = MAXN * MAXN, kCells = MAXN * MAXN * MAXN;
+    float p[kCells + 2 * kGuard + T];  // slack covers the unused tail slots
+    float wz[kCells + kGuard];         // scaled z+ couplings, zero guard plane below
+    float ap[kCells];                  // A p of the current iteration (own cells only)
+    float red[2][kWaves];
+    double redd[kWaves];
     double rhs_val[8];
     int rhs_idx[8];
     double enorm;
     int ticket;
 };
+using FdmLarge = FdmSharedT<1024, kMaxN, 1>;
+using FdmSmall = FdmSharedT<256, 16, 3>;  // registers capped so that three fit a CU
 
 typedef __attribute__((address_space(3))) float lds_float;
 
@@ -63,6 +67,7 @@ __device__ __forceinline__ lds_float* lds_base(float* p)
     return q;
 }
 
+template <int W>
 __device__ __forceinline__ float block_sum(float v, float* red)
 {
     v = wave_sum(v);
@@ -70,10 +75,11 @@ __device__ __forceinline__ float block_sum(float v, float* red)
     __syncthreads();
     float s = 0.f;
 #pragma unroll
-    for (int w = 0; w < kFdmWaves; w++) s += red[w];
+    for (int w = 0; w < W; w++) s += red[w];
     return s;
 }
 
+template <int W>
 __device__ __forceinline__ double block_sum_d(double v, double* red)
 {
     v = wave_sum(v);
@@ -81,7 +87,7 @@ __device__ __forceinline__ double block_sum_d(double v, double* red)
     __syncthreads();
     double s = 0.0;
 #pragma unroll
-    for (int w = 0; w < kFdmWaves; w++) s += red[w];
+    for (int w = 0; w < W; w++) s += red[w];
     __syncthreads();  // table free for the next call
     return s;
 }
@@ -147,16 +153,18 @@ __device__ void fdm_rhs(const float* E, int code, int* idx, double* val)
     for (int q = 0; q < 4; q++) { idx[4 + q] = minus[q]; val[4 + q] = D(minus[q]) * (1. / dsum + 1. / d_minus) / h * 2; }
 }
 
-template <int N>
-__device__ void fdm_cube(const FdmBatch& a, FdmShared& sh, int b)
+template <int N, typename SH>
+__device__ void fdm_cube(const FdmBatch& a, SH& sh, int b)
 {
+    static_assert(N <= SH::kMaxN, "cube larger than the size class");
     constexpr int NN = N * N, NNN = N * NN;
-    constexpr int T = kFdmThreads;
+    constexpr int T = SH::kThreads, W = SH::kWaves;
     constexpr int CPT = (NNN + T - 1) / T;
     const int tid = threadIdx.x;
     const float* E = a.eps + a.eps_off[b];
-    double* Y = a.y_ws + (size_t)blockIdx.x * kMaxCells;
-    float* S = a.s_ws + (size_t)blockIdx.x * kMaxCells;
+    // workspace slot of this workgroup (fdm_launch sizes the grids to fit)
+    double* Y = a.y_ws + (size_t)blockIdx.x * SH::kCells;
+    float* S = a.s_ws + (size_t)blockIdx.x * SH::kCells;
     const int code = a.code[b];
     float* P = sh.p + NN;  // valid for indices [-NN, NNN + NN)
 
@@ -245,7 +253,7 @@ __device__ void fdm_cube(const FdmBatch& a, FdmShared& sh, int b)
             loc += rv * rv;
             R32[c] = (float)rv * S[c];
         }
-        rnorm = sqrt(block_sum_d(loc, sh.redd));  // also orders the R32 writes
+        rnorm = sqrt(block_sum_d<W>(loc, sh.redd));  // also orders the R32 writes
         if (!(rnorm > a.tol * enorm)) break;      // converged (or enorm == 0)
 
         // ---- FP32 CG on the scaled system, right-hand side S r / ||r|| -------
@@ -259,7 +267,7 @@ __device__ void fdm_cube(const FdmBatch& a, FdmShared& sh, int b)
             rh[q] = (q < CPT - 1 || c < NNN) ? R32[c] * inv : 0.f;
             rloc += rh[q] * rh[q];
         }
-        float rr = block_sum(rloc, sh.red[1]);
+        float rr = block_sum<W>(rloc, sh.red[1]);
         double tgt = 0.3 * a.tol * enorm / rnorm;
         if (tgt < (double)a.inner_floor) tgt = a.inner_floor;
         const float thresh = (float)(tgt * tgt) * rr;
@@ -293,7 +301,7 @@ __device__ void fdm_cube(const FdmBatch& a, FdmShared& sh, int b)
                     pl += pc * v;
                 }
             }
-            const float pq = block_sum(pl, sh.red[0]);
+            const float pq = block_sum<W>(pl, sh.red[0]);
             const float alpha = rr / pq;
             float rl = 0.f;
 #pragma unroll
@@ -305,7 +313,7 @@ __device__ void fdm_cube(const FdmBatch& a, FdmShared& sh, int b)
                     rl += rh[q] * rh[q];
                 }
             }
-            const float rr_new = block_sum(rl, sh.red[1]);
+            const float rr_new = block_sum<W>(rl, sh.red[1]);
             total_it++;
             if (!(rr_new > thresh)) break;
             const float beta = rr_new / rr;
@@ -348,39 +356,49 @@ __device__ void fdm_cube(const FdmBatch& a, FdmShared& sh, int b)
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kFdmThreads) void fdm_pcg_kernel(FdmBatch a)
+__host__ __device__ constexpr bool fdm_n_ok(int n) { return n == 8 || n == 9 || n == 16 || n == 21 || n == 23; }
+
+// One persistent kernel per size class: workgroups take cubes by atomic
+// ticket (ctr[0]) and solve those of their class (supported N in
+// (NLO, SH::kMaxN]), skipping the rest; the last workgroup to leave resets
+// ctr for the next launch.  An unsupported N is marked (iters = -1) by
+// whichever class claims it.
+template <typename SH, int NLO>
+__global__ __launch_bounds__(SH::kThreads, SH::kPerCU) void fdm_pcg_kernel(FdmBatch a, uint32_t* ctr)
 {
-    __shared__ FdmShared sh;
+    __shared__ SH sh;
     for (;;) {
-        if (threadIdx.x == 0) sh.ticket = (int)atomicAdd(&a.counters[0], 1u);
+        if (threadIdx.x == 0) sh.ticket = (int)atomicAdd(&ctr[0], 1u);
         __syncthreads();
         const int t = sh.ticket;
         __syncthreads();
         if (t >= a.B) break;
         const int b = a.order ? a.order[t] : t;
-        switch (a.N[b]) {
-        case 8: fdm_cube<8>(a, sh, b); break;
-        case 9: fdm_cube<9>(a, sh, b); break;
-        case 16: fdm_cube<16>(a, sh, b); break;
-        case 21: fdm_cube<21>(a, sh, b); break;
-        case 23: fdm_cube<23>(a, sh, b); break;
+        const int n = a.N[b];
+        const bool mine = (NLO == 0 || n > NLO) && !(n > SH::kMaxN && fdm_n_ok(n));
+        if (!mine) continue;  // the other class (uniform across the workgroup)
+        switch (n) {
+        case 8: if constexpr (NLO < 8) fdm_cube<8>(a, sh, b); break;
+        case 9: if constexpr (NLO < 9) fdm_cube<9>(a, sh, b); break;
+        case 16: if constexpr (NLO < 16) fdm_cube<16>(a, sh, b); break;
+        case 21: if constexpr (SH::kMaxN >= 21) fdm_cube<21>(a, sh, b); break;
+        case 23: if constexpr (SH::kMaxN >= 23) fdm_cube<23>(a, sh, b); break;
         default:
             if (threadIdx.x == 0 && a.iters) a.iters[b] = -1;
             break;
         }
     }
-    // The last workgroup to leave resets the counters for the next launch.
     if (threadIdx.x == 0) {
         __threadfence();
-        const uint32_t done = atomicAdd(&a.counters[1], 1u);
+        const uint32_t done = atomicAdd(&ctr[1], 1u);
         if (done == gridDim.x - 1) {
-            atomicExch(&a.counters[0], 0u);
-            atomicExch(&a.counters[1], 0u);
+            atomicExch(&ctr[0], 0u);
+            atomicExch(&ctr[1], 0u);
         }
     }
 }
 
-bool fdm_supported_n(int n) { return n == 8 || n == 9 || n == 16 || n == 21 || n == 23; }
+bool fdm_supported_n(int n) { return fdm_n_ok(n); }
 
 int fdm_grid_size(int device)
 {
@@ -393,9 +411,24 @@ int fdm_launch(const FdmBatch& a, int grid, hipStream_t stream)
 {
     if (a.B <= 0) return 0;
     if (grid <= 0) return set_error(-2, "fdm_launch: grid must be positive");
-    if (grid > a.B) grid = a.B;
-    hipLaunchKernelGGL(fdm_pcg_kernel, dim3(grid), dim3(kFdmThreads), 0, stream, a);
-    DRW_HIP(hipGetLastError());
+    // grid = CUs; the workspace holds grid * kMaxCells cells
+    const bool small = a.n_hint == 0 || a.n_hint <= FdmSmall::kMaxN;
+    const bool large = a.n_hint == 0 || a.n_hint > FdmSmall::kMaxN;
+    if (large) {
+        const int g = grid < a.B ? grid : a.B;
+        hipLaunchKernelGGL((fdm_pcg_kernel<FdmLarge, FdmSmall::kMaxN>), dim3(g), dim3(FdmLarge::kThreads), 0, stream,
+                           a, a.counters);
+        DRW_HIP(hipGetLastError());
+    }
+    if (small) {
+        int g = FdmSmall::kPerCU * grid;
+        const int fit = (int)((size_t)grid * kMaxCells / FdmSmall::kCells);
+        if (g > fit) g = fit;
+        if (g > a.B) g = a.B;
+        hipLaunchKernelGGL((fdm_pcg_kernel<FdmSmall, 0>), dim3(g), dim3(FdmSmall::kThreads), 0, stream, a,
+                           a.counters + 2);
+        DRW_HIP(hipGetLastError());
+    }
     return 0;
 }
 
