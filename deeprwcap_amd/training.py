@@ -182,7 +182,8 @@ def _kl(pred_logp, target):
 
 
 def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log=print, seed=0, minutes=None,
-              eval_every=500, n_eval=512, loss_kind="kl", ema=0.999, init=None, best_path=None):
+              eval_every=500, n_eval=512, loss_kind="kl", ema=0.999, init=None, best_path=None,
+              face_sampling="uniform"):
     """Train one network.  With `minutes`, the run is time-boxed: the number
     of steps is set from the measured step time so that the cosine schedule
     ends when the budget does.  An exponential moving average of the weights
@@ -194,7 +195,13 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
     with every new best held-out state (a long run keeps its best so far).  `loss_kind` for the PoissonPredictor: "kl"
     (KL + 0.2 L1, the reference's main loss plus an L1 term) or "l1" (L1 +
     MSE of the normalised distribution: bounded gradients, and L1 is the
-    quantity the accuracy gate measures)."""
+    quantity the accuracy gate measures); for the PoissonSelector, "kl" (the
+    reference's loss) or "l1" (KL + L1 of the face probabilities).  `face_sampling` for the
+    PoissonPredictor: "uniform" (every face alike) or "prob" (a cube's face
+    is drawn with the probability the selector gives it, the FDM face sum:
+    the transition draws the predicted kernel with that probability, so
+    this weights the training loss and the held-out error as the
+    full-kernel error weights the faces)."""
     torch.manual_seed(seed)
     n = cubes.shape[0]
     ntr = int(n * 0.95)
@@ -234,7 +241,10 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
             tot = Gg[idx].abs().sum(-1)
             s = tot.sum(-1, keepdim=True)
             return x, torch.cat([tot / s, s], 1)
-        j = torch.randint(0, len(faces), (idx.shape[0],), device=dev)
+        if face_sampling == "prob" and name == "PoissonPredictor":
+            j = torch.multinomial(Gp[idx].sum(-1).clamp_min(0.0), 1).squeeze(1)
+        else:
+            j = torch.randint(0, len(faces), (idx.shape[0],), device=dev)
         fb = face_t[j]
         xr = torch.gather(x.reshape(x.shape[0], -1), 1, rot[j]).reshape(x.shape)
         if name == "PoissonPredictor":
@@ -248,7 +258,10 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
     def loss_of(x, tgt):
         out = model(x)
         if name == "PoissonSelector":
-            return _kl(F.log_softmax(out, -1), tgt)
+            lp = F.log_softmax(out, -1)
+            if loss_kind == "l1":
+                return _kl(lp, tgt) + (lp.exp() - tgt).abs().sum(-1).mean()
+            return _kl(lp, tgt)
         if name == "GradientSelectorWeight":
             p = F.softmax(out[:, :6], -1)
             return (_kl(F.log_softmax(out[:, :6], -1), tgt[:, :6]) + 0.2 * (p - tgt[:, :6]).abs().sum(-1).mean()
@@ -271,7 +284,7 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         m = avg.module if avg is not None else model
         m.eval()
         if name.endswith("Predictor"):
-            e = evaluate_face_l1(name, m, cubes, G, Gz, ev_idx, dev=dev)
+            e = evaluate_face_l1(name, m, cubes, G, Gz, ev_idx, dev=dev, weighted=face_sampling == "prob")
         else:
             with torch.no_grad():
                 x = Xall[ev_idx].float()
@@ -346,8 +359,10 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
     return model
 
 
-def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda"):
-    """Mean per-face normalised-kernel L1 on held-out cubes."""
+def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda", weighted=False):
+    """Mean per-face normalised-kernel L1 on held-out cubes; `weighted`
+    (PoissonPredictor): per cube, the faces' L1 weighted by their FDM face
+    probabilities, the predictor's share of the full-kernel L1."""
     faces = {"PoissonPredictor": range(6), "Gradient1Predictor": [0, 1], "Gradient2Predictor": [2, 3, 4, 5]}[name]
     Xall = torch.from_numpy(cubes[idx]).to(dev).float()
     errs = []
@@ -365,7 +380,10 @@ def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda"):
                 tgt = k / k.abs().sum(-1, keepdim=True)
                 y = model(_index_rot(Xall, f, "gradient")).reshape(len(idx), -1)
                 p = y / y.abs().sum(-1, keepdim=True)
-            errs.append((p - tgt).abs().sum(-1).mean().item())
+            e = (p - tgt).abs().sum(-1)
+            if weighted and name == "PoissonPredictor":
+                e = e * 6.0 * k.sum(-1) / torch.from_numpy(G[idx]).to(dev).sum((-1, -2))
+            errs.append(e.mean().item())
     return float(np.mean(errs))
 
 
@@ -384,11 +402,12 @@ SHARE = {"PoissonSelector": 0.12, "PoissonPredictor": 0.38, "GradientSelectorWei
          "Gradient2Predictor": 0.24, "Gradient1Predictor": 0.16}
 
 
-def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log, loss_kind="kl", ema=0.999, init=None):
+def train_one(out_dir, name, cubes, G, Gz, minutes, batch, lr, seed, log, loss_kind="kl", ema=0.999, init=None,
+              face_sampling="uniform"):
     os.makedirs(out_dir, exist_ok=True)
     m = train_net(name, np.ascontiguousarray(cubes), np.ascontiguousarray(G), np.ascontiguousarray(Gz), dev="cuda",
                   log=log, batch=batch, lr=lr, seed=seed, minutes=minutes, loss_kind=loss_kind, ema=ema, init=init,
-                  best_path=os.path.join(out_dir, f"{name}.drwnet"))
+                  best_path=os.path.join(out_dir, f"{name}.drwnet"), face_sampling=face_sampling)
     W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"), export_state(m))
     return m
 
@@ -443,8 +462,10 @@ if __name__ == "__main__":
                     help="wall-clock budget (sequential: for all five nets; --parallel/--only: per net)")
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--loss", default="kl", choices=("kl", "l1"), help="PoissonPredictor loss")
+    ap.add_argument("--loss", default="kl", choices=("kl", "l1"), help="PoissonPredictor / PoissonSelector loss")
     ap.add_argument("--ema", type=float, default=0.999, help="weight EMA decay (0: off)")
+    ap.add_argument("--face-sampling", default="uniform", choices=("uniform", "prob"),
+                    help="PoissonPredictor: faces drawn uniformly or with their FDM face probability")
     ap.add_argument("--make-data", metavar="PATH", help="only build the FDM dataset and save it under PATH.*.npy")
     ap.add_argument("--data", metavar="PATH", help="use a dataset saved by --make-data")
     ap.add_argument("--only", help="train this network only (needs --data)")
@@ -461,7 +482,7 @@ if __name__ == "__main__":
         _log(f"dataset: {args.n} cubes in {time.time() - t0:.1f}s -> {args.make_data}.*.npy")
     elif args.only:
         train_one(args.out, args.only, *load_dataset(args.data), args.minutes, args.batch, args.lr, args.seed, _log,
-                  loss_kind=args.loss, ema=args.ema, init=args.init)
+                  loss_kind=args.loss, ema=args.ema, init=args.init, face_sampling=args.face_sampling)
     elif args.parallel:
         _log(f"exported {train_parallel(args.out, args.data, args.minutes, args.batch, args.lr, args.seed, _log)}")
     else:
