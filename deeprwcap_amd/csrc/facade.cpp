@@ -243,10 +243,12 @@ void ParallelSampler::processBatch(std::vector<std::unique_ptr<Task>>& batch)
         return;
     }
     const size_t per = result.size() / batch.size();
-    for (size_t i = 0; i < batch.size(); ++i)
-        batch[i]->promise.set_value(std::vector<float>(result.begin() + i * per, result.begin() + (i + 1) * per));
+    // counted before the results are handed out, so that a caller holding
+    // every result also sees them in stats()
     tasks_ += batch.size();
     batches_ += 1;
+    for (size_t i = 0; i < batch.size(); ++i)
+        batch[i]->promise.set_value(std::vector<float>(result.begin() + i * per, result.begin() + (i + 1) * per));
 }
 
 ParallelSampler::Stats ParallelSampler::stats() const

@@ -6,7 +6,7 @@
 // header recompiles unchanged (binary layout differs).
 //
 // Additions (no counterpart in the reference): handle(), useFDM(),
-// setSeed(), ParallelSampler::stats().
+// setSeed(), ParallelSampler::stats(), the NoDevice constructor.
 #pragma once
 
 #include <atomic>
@@ -68,6 +68,12 @@ public:
 
 protected:
     DNNSolver(int kind, int device_idx);
+    // For subclasses that implement both sample() overloads themselves and
+    // need no device solver (a recorded or remote sampler, a test double in
+    // front of ParallelSampler): no handle is created, and the handle-based
+    // calls (useFDM, setSeed, loadModelsFrom, run) throw.
+    struct NoDevice {};
+    DNNSolver(NoDevice, int kind) : kind_(kind) {}
     void loadModelsFrom(const std::string& models_txt, bool verbose);
     std::vector<float> run(const std::vector<float>& layers, const std::vector<float>& cuboids, const int32_t* axis);
 
