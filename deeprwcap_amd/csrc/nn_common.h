@@ -169,24 +169,42 @@ __device__ __forceinline__ float gelu(float x)
     return x >= 0.f ? x - xq : xq;
 }
 
-// GELU by table: Phi(x) is linear between 513 nodes on [-6, 6] (h = 12/512,
-// interpolation error <= h^2/8 max|Phi''| = 1.7e-5), read from a 512-entry
-// (Phi(x_i), Phi(x_i+1) - Phi(x_i)) table in LDS; outside [-6, 6] the end
-// entries give x (1 - 1e-9) and x 1e-9.  7 VALU instructions and one LDS
-// read.  The face predictors use it for every layer: on the fp32 tail too,
-// the network's error against the fp32 forward stays within its fp16
-// budget (CPU emulation of the kernel's rounding points).  The host builds
-// the table once per network in double precision (nn_host.cpp).
+// GELU by table: Phi(x) is linear between NTAB + 1 nodes on [lo, -lo], read
+// from an NTAB-entry (Phi(x_i), Phi(x_i+1) - Phi(x_i)) table in LDS; outside
+// the range the end entries give x Phi(-lo) and x Phi(lo).  7 VALU
+// instructions and one LDS read.  The host builds the tables once per network
+// in double precision (nn_host.cpp, gelu_table).
+//  * face predictors, every layer: 512 entries on [-6, 6] (h = 12/512,
+//    interpolation error <= h^2/8 max|Phi''| = 1.7e-5); on the fp32 tail
+//    too, the network's error against the fp32 forward stays within its fp16
+//    budget (CPU emulation of the kernel's rounding points);
+//  * selectors, first conv: 216 entries on [-4.5, 4.5] (error <= 4.5e-5 in
+//    GELU, tails x Phi(-4.5) = 3.4e-6 x), the size of the LDS the im2col
+//    offsets of the later layers use, which is free during the first; those
+//    outputs are stored as fp16 (relative rounding 4.9e-4).
 constexpr int kGeluTab = 512;
 constexpr float kGeluLo = -6.0f, kGeluInvH = kGeluTab / 12.0f;
+constexpr int kSelGeluTab = 216;
+constexpr float kSelGeluLo = -4.5f, kSelGeluInvH = kSelGeluTab / 9.0f;
 
-__device__ __forceinline__ float gelu_tab(float x, const float2* tab)
+template <int NTAB>
+__device__ __forceinline__ float gelu_lut(float x, const float2* tab, float inv_h, float lo)
 {
-    float u = fmaf(x, kGeluInvH, -kGeluLo * kGeluInvH);
-    u = __builtin_amdgcn_fmed3f(u, 0.0f, (float)kGeluTab - 6.1e-5f);
+    float u = fmaf(x, inv_h, -lo * inv_h);
+    u = __builtin_amdgcn_fmed3f(u, 0.0f, (float)NTAB - 6.1e-5f);
     const int i = (int)u;
     const float2 e = tab[i];
     return x * fmaf(e.y, __builtin_amdgcn_fractf(u), e.x);
+}
+
+__device__ __forceinline__ float gelu_tab(float x, const float2* tab)
+{
+    return gelu_lut<kGeluTab>(x, tab, kGeluInvH, kGeluLo);
+}
+
+__device__ __forceinline__ float gelu_tab_sel(float x, const float2* tab)
+{
+    return gelu_lut<kSelGeluTab>(x, tab, kSelGeluInvH, kSelGeluLo);
 }
 
 // MFMA 16x16x16 fp16 -> fp32.  A: lane l holds row l%16, k = 4(l/16)..+3;

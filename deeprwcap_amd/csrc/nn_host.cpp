@@ -155,12 +155,12 @@ std::vector<half_t> pack_frag_input(const std::vector<float>& W, int N)
 
 // Table of gelu_tab: (Phi(x_i), Phi(x_i+1) - Phi(x_i)) at the nodes
 // x_i = -6 + i h of [-6, 6], Phi(x) = erfc(-x / sqrt 2) / 2 in double.
-std::vector<float2> gelu_table()
+std::vector<float2> gelu_table(int n = kGeluTab, double lo = kGeluLo, double inv_h = kGeluInvH)
 {
-    std::vector<float2> t(kGeluTab);
+    std::vector<float2> t(n);
     auto phi = [](double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); };
-    for (int i = 0; i < kGeluTab; i++) {
-        const double x0 = kGeluLo + (double)i / kGeluInvH, x1 = kGeluLo + (double)(i + 1) / kGeluInvH;
+    for (int i = 0; i < n; i++) {
+        const double x0 = lo + (double)i / inv_h, x1 = lo + (double)(i + 1) / inv_h;
         const double p0 = phi(x0);
         t[i] = make_float2((float)p0, (float)(phi(x1) - p0));
     }
@@ -280,6 +280,7 @@ int build_sel(const TensorMap& m, bool weighted, Arena& ar, SelNetDev& d)
         ar.add(pad(b, 64), &d.bias[l]);
         if (l == 0) ar.add(conv1_pairs(W, co), &d.conv1_w);
     }
+    ar.add(gelu_table(kSelGeluTab, kSelGeluLo, kSelGeluInvH), &d.gtab);
     const std::vector<int> dims = weighted ? std::vector<int>{512, 256, 128, 7} : std::vector<int>{128, 64, 6};
     for (size_t j = 0; j + 1 < dims.size(); j++) {
         const std::string k = "head." + std::to_string(2 * j + 1);

@@ -13,6 +13,7 @@ struct SelNetDev {
     const half4_t* frag[6];
     const float* bias[6];
     const half_t* conv1_w;  // first Conv3d for the direct form: per channel 14 fp16 tap pairs (conv1_pairs)
+    const float2* gtab;     // GELU table of the first Conv3d (kSelGeluTab entries)
 };
 
 struct SelBackboneArgs {

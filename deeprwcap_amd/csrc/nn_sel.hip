@@ -114,7 +114,7 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
     __syncthreads();
 }
 
-// First Conv3d (1 -> C1, k3, s2, no padding) + folded BN + GELU on the VALU,
+// First Conv3d (1 -> C1, k3, s2, no padding) + folded BN + GELU (gelu_tab_sel) on the VALU,
 // one output z-slab per phase, in place: slab oz reads input planes
 // 2oz..2oz+2, so after the phase's barrier the planes below 2oz+2 are dead
 // and the slab's outputs (held in registers until then) are written over
@@ -131,7 +131,7 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 // implicit-GEMM form spends most of its instructions on the gather and on
 // moving the valid columns out of the 16-wide MFMA tile.
 template <int C1>
-__device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, const float* bias)
+__device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, const float* bias, const float2* gtab)
 {
     constexpr int DO = 11, DO2 = DO * DO, CP = C1 / 2, NP = 14;
     constexpr int IPT = (DO2 * CP + 255) / 256;  // items per thread and phase
@@ -173,8 +173,8 @@ __device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, c
                     a1 = __builtin_amdgcn_fdot2(sp, w1[9 + k], a1, false);
                 }
             }
-            held[u][0] = gelu(a0);
-            held[u][1] = gelu(a1);
+            held[u][0] = gelu_tab_sel(a0, gtab);
+            held[u][1] = gelu_tab_sel(a1, gtab);
         }
         __syncthreads();  // this slab's input planes fully read
         half_t* o = out + oz * (C1 * DO2);
@@ -197,7 +197,8 @@ __device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, c
 template <int C1, int C2>
 struct SelSmem {
     static constexpr int kRegion = kNNN > C1 * 1331 + C2 * 125 ? kNNN : C1 * 1331 + C2 * 125;
-    alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4)
+    alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4);
+                                    //           the first conv's GELU table before that
     half_t cube[kRegion];           // >= 24,334 B
     VoxDesc desc;
 };
@@ -205,11 +206,16 @@ struct SelSmem {
 template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs a, SelNetDev w)
 {
+    static_assert(sizeof(SelSmem<4, 8>::koff) == kSelGeluTab * sizeof(float2), "GELU table fills the offset slot");
     constexpr int C1 = WEIGHTED ? 8 : 4, C2 = WEIGHTED ? 16 : 8, C3 = WEIGHTED ? 64 : 16;
     static_assert(C1 * 1331 + C2 * 125 <= SelSmem<C1, C2>::kRegion, "conv outputs must fit the shared region");
     __shared__ SelSmem<C1, C2> s;
     const int b = blockIdx.x;
+    // wave 0 builds the description; the others stage the first conv's GELU
+    // table in the im2col-offset slot, which only the later convs use
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
+    else
+        for (int i = threadIdx.x - 64; i < kSelGeluTab; i += 192) ((float2*)s.koff)[i] = w.gtab[i];
     __syncthreads();
     const int axis = a.axis ? a.axis[b] : 2;
     // Cube in the (axis pre-rotated) frame, built row by row: along an
@@ -247,7 +253,7 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
     __syncthreads();
     half_t* l1 = s.cube;                 // [11][C1][121], over the cube
     half_t* l2 = s.cube + C1 * 1331;     // [C2][125], after l1
-    conv1_direct<C1>(s.cube, l1, w.conv1_w, w.bias[0]);
+    conv1_direct<C1>(s.cube, l1, w.conv1_w, w.bias[0], (const float2*)s.koff);
     conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false>(l1, l2, w.frag[1], w.bias[1], s.koff);
     conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, true>(l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2], s.koff);
 }
