@@ -174,12 +174,14 @@ __device__ __forceinline__ float gelu(float x)
 // the range the end entries give x Phi(-lo) and x Phi(lo).  7 VALU
 // instructions and one LDS read.  The host builds the tables once per network
 // in double precision (nn_host.cpp, gelu_table).
+// The interpolation error of x Phi(x) is at most h^2/8 max x^2 phi(x) =
+// 0.294 h^2/8.
 //  * face predictors, every layer: 512 entries on [-6, 6] (h = 12/512,
-//    interpolation error <= h^2/8 max|Phi''| = 1.7e-5); on the fp32 tail
-//    too, the network's error against the fp32 forward stays within its fp16
-//    budget (CPU emulation of the kernel's rounding points);
-//  * selectors, first conv: 216 entries on [-4.5, 4.5] (error <= 4.5e-5 in
-//    GELU, tails x Phi(-4.5) = 3.4e-6 x), the size of the LDS the im2col
+//    error <= 2.0e-5); on the fp32 tail too, the network's error against the
+//    fp32 forward stays within its fp16 budget (CPU emulation of the
+//    kernel's rounding points);
+//  * selectors, first conv: 216 entries on [-4.5, 4.5] (h = 9/216, error <=
+//    6.4e-5, tails x Phi(-4.5) = 3.4e-6 x), the size of the LDS the im2col
 //    offsets of the later layers use, which is free during the first; those
 //    outputs are stored as fp16 (relative rounding 4.9e-4).
 constexpr int kGeluTab = 512;
