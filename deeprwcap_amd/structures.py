@@ -86,6 +86,35 @@ def layered_boxes(B: int, N, seed: int = 0, max_boxes: int = 2, eps_lo: float = 
     return layers, cuboids, Ns
 
 
+def full_descriptions(B: int, N: int, seed: int = 0, eps_lo: float = 1.0, eps_hi: float = 8.0):
+    """Descriptions at the format's limits: all 15 z-layers (the first from
+    below the cube, the other 14 on distinct grid faces of the N-cell cube,
+    sorted) and all 15 cuboids, none padded.  Boxes overlap (first match
+    wins), some reach past the cube faces and some are one cell thin.
+    Needs N >= 15."""
+    if N < MAX_LAYERS:
+        raise ValueError("full_descriptions needs N >= 15 (14 distinct interfaces)")
+    rng = np.random.default_rng(seed)
+    layers, cuboids = empty(B)
+    for b in range(B):
+        ifaces = np.sort(rng.choice(np.arange(1, N), size=MAX_LAYERS - 1, replace=False))
+        eps = rng.uniform(eps_lo, eps_hi, size=MAX_LAYERS).astype(np.float32)
+        layers[b, 0] = (-SPAN, eps[0])
+        for l in range(1, MAX_LAYERS):
+            layers[b, l] = (_grid_coord(ifaces[l - 1], N), eps[l])
+        for c in range(MAX_CUBOIDS):
+            lo = rng.integers(0, N, size=3)
+            hi = np.array([rng.integers(lo[d] + 1, min(N, lo[d] + 1 + (1 if c % 5 == 0 else N)) + 1)
+                           for d in range(3)])
+            box = [_grid_coord(lo[d], N) for d in range(3)] + [_grid_coord(hi[d], N) for d in range(3)]
+            if c % 4 == 1:  # reaches past the cube on one axis
+                d = int(rng.integers(0, 3))
+                box[d], box[d + 3] = np.float32(-SPAN), np.float32(SPAN)
+            cuboids[b, c, 0:6] = box
+            cuboids[b, c, 6] = np.float32(rng.uniform(eps_lo, eps_hi))
+    return layers, cuboids
+
+
 def mixed_N(B: int, seed: int = 0, choices=(16, 21, 23)):
     """N drawn uniformly from ``choices`` (deterministic in seed)."""
     rng = np.random.default_rng(seed ^ 0x5EED)

@@ -129,6 +129,18 @@ def test_voxelize_bitexact(dev, orc, N):
         np.testing.assert_array_equal(mx.cpu().numpy(), rmx)
 
 
+@pytest.mark.parametrize("N", [16, 23])
+def test_voxelize_full_descriptions(dev, orc, N):
+    """All 15 layers and all 15 cuboids in use (no padding), overlapping."""
+    from deeprwcap_amd import structures as S
+    L, C = S.full_descriptions(32, N, seed=100 + N)
+    for norm in (True, False):
+        cube, mx = dev.voxelize(L, C, fixed_N=N, normalise=norm)
+        ref, rmx = orc.voxelize(L, C, N, normalise=norm)
+        np.testing.assert_array_equal(cube.cpu().numpy(), ref)
+        np.testing.assert_array_equal(mx.cpu().numpy(), rmx)
+
+
 def test_sample_flat_bitexact(dev, orc):
     from deeprwcap_amd import structures as S
     import torch

@@ -72,6 +72,27 @@ def test_nn_kernels_match_fp32(kind, models_txt, params):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
+def test_full_descriptions_match_fp32(kind, models_txt, params):
+    """Descriptions at the format's limits (15 layers, 15 overlapping
+    cuboids, no padding) through the fused in-kernel voxelization."""
+    from deeprwcap_amd import structures as S
+    from oracle import nets_ref as R
+    B = 24
+    L, C = S.full_descriptions(B, 23, seed=200 + kind)
+    axis = (np.arange(B) % 3).astype(np.int32) if kind == 1 else None
+    s = _solver(kind, models_txt)
+    probs, wts, kern = _eval(s, L, C, axis)
+    rp, rw, rk = R.query_kernels(kind, L, C, axis, params)
+    l1 = np.abs(kern - rk).sum(-1)
+    print(f"full descriptions kind {kind}: probs max err {np.abs(probs - rp).max():.2e}, kernel L1 max "
+          f"{l1.max():.2e}", flush=True)
+    assert np.abs(probs - rp).max() < 2e-3
+    assert l1.max() < L1_TOL
+    if kind == 1:
+        assert np.abs(wts - rw).max() < 2e-3 * max(1.0, np.abs(rw).max())
+
+
+@pytest.mark.parametrize("kind", [0, 1])
 def test_trained_weights_match_fp32(kind):
     """The same parity on the shipped weights (models/): trained kernels are
     peaked, so the fp16 activations' error shows far more than on seeded
