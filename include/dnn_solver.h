@@ -47,7 +47,9 @@ public:
     DNNSolver(const DNNSolver&) = delete;
     DNNSolver& operator=(const DNNSolver&) = delete;
 
-    // Offline readers (dnn_solver.cpp:89-250)
+    // Offline readers (dnn_solver.cpp:89-250).  Errors throw
+    // std::runtime_error: a file that cannot be opened, and (readBinaryFile)
+    // one that ends before numSamples records.
     static void readBinaryFile(const std::string& filePath, std::vector<float>& dielectricConfigs,
                                std::vector<float>& targetFunctions, int numSamples, int structureSize,
                                DataType inputType, bool verbose);
