@@ -3,8 +3,10 @@
 // copied.  TEST INFRASTRUCTURE ONLY: it writes golden vectors and times the
 // reference CPU path; no product code uses it.
 //
-// The reference fixes N = 23 at compile time (ggft/src/ggft_solver.h:33), so
-// this driver covers N = 23 only.
+// The reference fixes N at compile time (ggft/src/ggft_solver.h:33): the
+// driver takes N from the header it is compiled against (oracle/Makefile
+// builds it at N = 23 from the sources as they are and at N = 16, 21 from
+// copies in which only that constant differs).
 //
 // stdin : int32 count, int32 code (0 GFT, 1 WVTZ, 2 WVTX, 3 WVTY),
 //         float64 tol, then count cubes of N^3 float64 permittivities

@@ -12,8 +12,12 @@ keyed like the reference state dicts:
   SignedFacePredictor architecture.py:135-173 channel_dim = 1
   FaceSelector       architecture.py:176-205 (PoissonSelector)
   FaceSelectorWeight architecture.py:208-240 (GradientSelectorWeight)
-Pinned by tests/golden/nets_golden.npz (outputs of the reference modules on
-seeded weights, written by tests/golden/make_net_golden.py).
+Pinned by tests/test_oracle_nets.py against the outputs of the reference
+modules themselves on seeded weights (tests/golden/nets_golden.npz and the
+high-contrast tests/golden/nets_wiring.npz, written by
+tests/golden/make_net_golden.py): exact match, and each planted wiring slip
+(swapped position channels, missing permute, wrong head, reordered
+dilations) moves the wiring fixture's outputs by 16 % or more.
 """
 from __future__ import annotations
 

@@ -19,7 +19,18 @@ def orc():
     return O
 
 
+def load_golden_ggft(N):
+    """GGFT golden vectors of the reference solver at resolution N (16, 21, 23),
+    written by tests/golden/make_ggft_golden.py."""
+    import numpy as np
+    return dict(np.load(os.path.join(ROOT, "tests", "golden", f"ggft_n{N}.npz")))
+
+
 @pytest.fixture(scope="session")
 def golden_ggft():
-    import numpy as np
-    return dict(np.load(os.path.join(ROOT, "tests", "golden", "ggft_n23.npz")))
+    return load_golden_ggft(23)
+
+
+@pytest.fixture(scope="session", params=[16, 21, 23], ids=lambda n: f"n{n}")
+def golden_ggft_n(request):
+    return load_golden_ggft(request.param)

@@ -1,8 +1,9 @@
 """GPU parity of the batched FDM kernel, voxelizer and flat sampler (C ABI).
 
 Compares the HIP path with the oracle / golden fixtures on the same inputs:
-* GGFT golden vectors (reference solver built from its sources, N = 23):
-  FP64 mode (tol 1e-12) within 1e-10 relative, refinement to 1e-8 within 1e-6;
+* GGFT golden vectors (reference solver built from its sources, N = 16, 21,
+  23, all four right-hand sides): FP64 mode (tol 1e-12) within 1e-10
+  relative, refinement to 1e-8 within 1e-6;
 * mixed N in {8, 9, 16, 21, 23} and all four right-hand sides against the
   FP64 oracle;
 * full size (4096 cubes, N mixed in {16, 21, 23}): size-independent
@@ -35,8 +36,8 @@ def _solve(dev, cubes, Ns, codes, tol):
     return pk, G, iters.cpu().numpy(), rr.cpu().numpy()
 
 
-def test_fdm_golden_ggft(dev, orc, golden_ggft):
-    g = golden_ggft
+def test_fdm_golden_ggft(dev, orc, golden_ggft_n):
+    g = golden_ggft_n
     N = int(g["N"])
     cubes, _ = orc.voxelize(g["layers"], g["cuboids"], N, normalise=False)
     B = len(g["codes"])
@@ -44,7 +45,7 @@ def test_fdm_golden_ggft(dev, orc, golden_ggft):
         pk, G, it, rr = _solve(dev, list(cubes), [N] * B, g["codes"], tol)
         for b in range(B):
             k = G[pk.G_off[b]:pk.G_off[b] + 6 * N * N]
-            print(f"golden case {b} code {g['codes'][b]} tol {tol}: iters {it[b]} relres {rr[b]:.2e} "
+            print(f"golden N {N} case {b} code {g['codes'][b]} tol {tol}: iters {it[b]} relres {rr[b]:.2e} "
                   f"err {_relerr(k, g['G'][b]):.2e}", flush=True)
             assert it[b] > 0 and rr[b] <= tol
             assert _relerr(k, g["G"][b]) < bound

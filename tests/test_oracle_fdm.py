@@ -5,7 +5,8 @@ face weight, a dropped boundary term, a transposed face layout, a wrong
 even-N source) fails at least one of them:
 
 * GGFT golden vectors from the reference solver compiled from its sources
-  (N = 23, GFT / WVTZ / WVTX / WVTY, tolerance 1e-13);
+  (N = 16, 21 and 23, GFT / WVTZ / WVTX / WVTY, tolerance 1e-13; N = 16
+  takes the reference's even-N source and gradient stencils);
 * homogeneous eps: separable DST-II closed form, odd and even N;
 * z-layered eps: DST-II in x, y and a tridiagonal solve per mode in z;
 * tiny N: a dense solve of GGFT's own assembled matrix A = L D^-1;
@@ -70,8 +71,8 @@ def _relerr(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
-def test_golden_ggft_n23(orc, golden_ggft):
-    g = golden_ggft
+def test_golden_ggft(orc, golden_ggft_n):
+    g = golden_ggft_n
     N = int(g["N"])
     cubes, _ = orc.voxelize(g["layers"], g["cuboids"], N, normalise=False)
     for b in range(len(g["codes"])):
