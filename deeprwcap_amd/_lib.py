@@ -35,6 +35,7 @@ SIGNATURES = {
     "drw_solver_stage_times": (_c_int, [_c_vp, _c_vp, _c_vp, _c_int]),
     "drw_solver_sample": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_device": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
+    "drw_solver_sample_device_noalloc": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_sample_ids": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_solver_graph_create": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, ctypes.POINTER(_c_vp)]),
     "drw_graph_launch": (_c_int, [_c_vp, _c_vp]),

@@ -13,53 +13,9 @@
 #include "kernels.h"
 #include "query.h"
 #include "runtime.h"
+#include "solver.h"
 
 using namespace drw;
-
-namespace {
-
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    int ensure(size_t want)
-    {
-        if (want <= n) return 0;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        const size_t cap = want + want / 4 + 16;
-        DRW_HIP(hipMalloc(&p, sizeof(T) * cap));
-        n = cap;
-        return 0;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-}  // namespace
-
-struct drw_solver {
-    int kind = DRW_POISSON;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int backend = -1;  // none until load_models / use_fdm
-    int fdm_N = 23;
-    double fdm_tol = 1e-6;
-    uint64_t seed = 0x5eed5eedULL;
-    uint64_t counter = 0;
-    drw::NetWeights* nets = nullptr;
-    DevBuf<float> layers, cuboids, out, eps, cen;
-    DevBuf<int32_t> axis, Nq, code;
-    DevBuf<int64_t> eps_off, G_off;
-    DevBuf<double> G;
-    DevBuf<uint8_t> scratch;
-    StageTimer timer;
-};
 
 extern "C" {
 
@@ -132,6 +88,8 @@ int drw_solver_create(int kind, int device_idx, drw_solver** out)
     if (device_idx < 0 || device_idx >= ndev)
         return set_error(DRW_EINVAL, "drw_solver_create: device %d out of range (%d devices)", device_idx, ndev);
     DRW_HIP(hipSetDevice(device_idx));
+    int rc = noalloc_scratch_init(device_idx);
+    if (rc) return rc;
     drw_solver* s = new drw_solver();
     s->kind = kind;
     s->device = device_idx;

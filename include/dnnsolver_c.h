@@ -154,6 +154,18 @@ int drw_solver_sample_ex(drw_solver* s, int B, const float* layers, const float*
                          float* out, int32_t* face_out, float* probs_out, float* weight_out, float* kern_out,
                          void* stream);
 
+/* Allocation-free drw_solver_sample_device.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels, results and counter use as drw_solver_sample_device; the per-query
+ * intermediates live in a fixed 16.25 MiB device array that the library holds
+ * on each GPU it runs on until the process ends, so no call allocates or frees
+ * anything.  Network backend only (DRW_ESTATE otherwise); B <= 8192
+ * (DRW_EINVAL beyond).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, threads or solvers) race. */
+int drw_solver_sample_device_noalloc(drw_solver* s, int B, const float* layers, const float* cuboids,
+                                     const int32_t* axis, float* out, void* stream);
+
 /* Full transition kernels from the networks (no draws), for accuracy checks
  * against the FDM reference: probs_out [B][6], kern_out [B][6][529] (face f of
  * the query's frame, tile t = 23 b + a in GGFT panel order), weight_out [B]
