@@ -169,23 +169,28 @@ __device__ __forceinline__ float gelu(float x)
     return x >= 0.f ? x - xq : xq;
 }
 
-// GELU by table: Phi(x) is linear between NTAB + 1 nodes on [lo, -lo], read
-// from an NTAB-entry (Phi(x_i), Phi(x_i+1) - Phi(x_i)) table in LDS; outside
-// the range the end entries give x Phi(-lo) and x Phi(lo).  7 VALU
-// instructions and one LDS read.  The host builds the tables once per network
-// in double precision (nn_host.cpp, gelu_table).
-// The interpolation error of x Phi(x) is at most h^2/8 max x^2 phi(x) =
-// 0.294 h^2/8.
-//  * face predictors, every layer: 512 entries on [-6, 6] (h = 12/512,
-//    error <= 2.0e-5); on the fp32 tail too, the network's error against the
-//    fp32 forward stays within its fp16 budget (CPU emulation of the
-//    kernel's rounding points);
-//  * selectors, first conv: 216 entries on [-4.5, 4.5] (h = 9/216, error <=
-//    6.4e-5, tails x Phi(-4.5) = 3.4e-6 x), the size of the LDS the im2col
-//    offsets of the later layers use, which is free during the first; those
-//    outputs are stored as fp16 (relative rounding 4.9e-4).
+// GELU by table, two forms.
+//  * Face predictors, every layer (gelu_tab): Phi is linear in x on each of
+//    kGeluTab intervals of [kGeluLo, -kGeluLo] = [-5, 5]; entry i + 1 holds
+//    the line (a, b) of interval i, Phi(x) ~ a + b x, placed midway between
+//    the largest deviations of Phi above and below its chord on that interval
+//    (the host builds it in double precision, nn_host.cpp, phi_line_table),
+//    and GELU(x) = x (a + b x).  Entry 0 is (0, 0) for x < -5 and entry
+//    kGeluTab + 1 is (1, 0) for x >= 5, where |GELU(x) - max(x, 0)| =
+//    |x| Phi(-|x|) <= 1.5e-6.  The error is |x| times that of the line,
+//    <= 7.1e-6 on all of R (largest near |x| = 1.4, about 2e-6 on average
+//    over N(0, 1) inputs), in 6 VALU instructions and one LDS read.
+//  * Selectors, first conv (gelu_tab_sel): Phi(x) is linear between NTAB + 1
+//    nodes on [lo, -lo], read from an NTAB-entry (Phi(x_i), Phi(x_i+1) -
+//    Phi(x_i)) table; outside the range the end entries give x Phi(-lo) and
+//    x Phi(lo).  216 entries on [-4.5, 4.5] (h = 9/216, error <= h^2/8 max
+//    x^2 phi(x) = 0.294 h^2/8 = 6.4e-5, tails x Phi(-4.5) = 3.4e-6 x), the
+//    size of the LDS the im2col offsets of the later layers use, which is
+//    free during the first; those outputs are stored as fp16 (relative
+//    rounding 4.9e-4).
 constexpr int kGeluTab = 512;
-constexpr float kGeluLo = -6.0f, kGeluInvH = kGeluTab / 12.0f;
+constexpr int kGeluTabEntries = kGeluTab + 2;
+constexpr float kGeluLo = -5.0f, kGeluInvH = kGeluTab / 10.0f;
 constexpr int kSelGeluTab = 216;
 constexpr float kSelGeluLo = -4.5f, kSelGeluInvH = kSelGeluTab / 9.0f;
 
@@ -201,7 +206,12 @@ __device__ __forceinline__ float gelu_lut(float x, const float2* tab, float inv_
 
 __device__ __forceinline__ float gelu_tab(float x, const float2* tab)
 {
-    return gelu_lut<kGeluTab>(x, tab, kGeluInvH, kGeluLo);
+    // u in [1, kGeluTab + 1) on the table's range; below it entry 0, above it
+    // entry kGeluTab + 1
+    float u = fmaf(x, kGeluInvH, 1.0f - kGeluLo * kGeluInvH);
+    u = __builtin_amdgcn_fmed3f(u, 0.0f, (float)kGeluTab + 1.5f);
+    const float2 e = tab[(int)u];
+    return x * fmaf(e.y, x, e.x);
 }
 
 __device__ __forceinline__ float gelu_tab_sel(float x, const float2* tab)

@@ -27,9 +27,10 @@ namespace drw {
 // the A operand, activations as B; the packed fragments are the same
 // registers in either role), so a lane ends with 4 consecutive channels of
 // one pixel and writes them with one 8-byte store.  The weights enter in two
-// fp16 parts (frag = fp16(W), frag_lo = W - fp16(W)) accumulated in separate
-// chains: with fp16 weights alone the FaceSolverPlus kernels come within a
-// few percent of the 1e-3 L1 bound against the fp32 forward.
+// fp16 parts (frag = fp16(W), frag_lo = W - fp16(W)): with fp16 weights
+// alone the FaceSolverPlus kernels come within a few percent of the 1e-3 L1
+// bound against the fp32 forward.  Both parts accumulate in one chain that
+// starts from the bias, so the epilogue is the GELU alone.
 template <int CIN_PAD, int COUT, int COUT_PAD, int CSI, int CSO>
 __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const half4_t* frag_lo, const float* bias,
                         const float2* gtab)
@@ -38,7 +39,7 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = 4 * (lane >> 4);
     half4_t bw[NT][KT], bl[NT][KT];
-    float bb[NT][4];
+    float4_t bb[NT];  // bias of the accumulator rows (channels kq..kq+3): the chains start from it
 #pragma unroll
     for (int n = 0; n < NT; n++) {
 #pragma unroll
@@ -46,11 +47,8 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
             bw[n][k] = frag[(n * KT + k) * 64 + lane];
             bl[n][k] = frag_lo[(n * KT + k) * 64 + lane];
         }
-        const int V = COUT - n * 16 < 16 ? COUT - n * 16 : 16;
-        // full tile: channels kq..kq+3; half tile (see below): two channels
-        const int c0 = V == 16 ? kq : ((lane >> 5) ? 2 : 0) + 4 * ((lane >> 4) & 1);
 #pragma unroll
-        for (int i = 0; i < 4; i++) bb[n][i] = (V == 16 || i < 2) ? bias[n * 16 + c0 + i] : 0.f;
+        for (int i = 0; i < 4; i++) bb[n][i] = n * 16 + kq + i < COUT ? bias[n * 16 + kq + i] : 0.f;
     }
     for (int m = wave; m < kPix / 16; m += 4) {
         half4_t a[KT];
@@ -59,25 +57,24 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
         half_t* orow = out + (m * 16 + r) * CSO;
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            float4_t acc = {0.f, 0.f, 0.f, 0.f}, acl = {0.f, 0.f, 0.f, 0.f};
+            // one chain: bias, then the small W - fp16(W) part, then fp16(W)
+            float4_t acc = bb[n];
             if constexpr (KT == 2) {
                 const half8_t a8 = cat8(a[0], a[1]);
-                acl = mfma32(cat8(bl[n][0], bl[n][1]), a8, acl);
+                acc = mfma32(cat8(bl[n][0], bl[n][1]), a8, acc);
                 acc = mfma32(cat8(bw[n][0], bw[n][1]), a8, acc);
             } else {
 #pragma unroll
-                for (int k = 0; k < KT; k++) {
-                    acl = mfma16(bl[n][k], a[k], acl);
-                    acc = mfma16(bw[n][k], a[k], acc);
-                }
+                for (int k = 0; k < KT; k++) acc = mfma16(bl[n][k], a[k], acc);
+#pragma unroll
+                for (int k = 0; k < KT; k++) acc = mfma16(bw[n][k], a[k], acc);
             }
-            acc += acl;
             constexpr int VFULL = 16;
             const int V = COUT - n * 16 < VFULL ? COUT - n * 16 : VFULL;  // valid channels of this tile
             if (V == VFULL) {
                 half4_t o;
 #pragma unroll
-                for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i] + bb[n][i], gtab);
+                for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i], gtab);
                 *(half4_t*)(orow + n * 16 + kq) = o;
             } else {
                 static_assert(COUT % 16 == 0 || COUT % 16 == 8, "tiles hold 16 or 8 valid channels");
@@ -88,8 +85,8 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
                 const int c0 = (hi ? 2 : 0) + 4 * ((lane >> 4) & 1);
                 typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
                 h2_t o;
-                o[0] = (half_t)gelu_tab((hi ? t2 : acc[0]) + bb[n][0], gtab);
-                o[1] = (half_t)gelu_tab((hi ? t3 : acc[1]) + bb[n][1], gtab);
+                o[0] = (half_t)gelu_tab(hi ? t2 : acc[0], gtab);
+                o[1] = (half_t)gelu_tab(hi ? t3 : acc[1], gtab);
                 *(h2_t*)(orow + n * 16 + c0) = o;
             }
         }
@@ -124,26 +121,69 @@ __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float
 
 // Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU, one
 // output (pixel, channel pair) per item with all 9 taps read from LDS: every
-// item is independent, so all four waves stay busy.  The taps are
-// branch-free: a tap outside the image reads `zero` (a zeroed LDS slot)
-// instead, so the nine loads of an item issue back to back.  Channel pairs
-// run fastest within a wave, so each tap is one contiguous 64-lane read.
-// Needs 256 % (C/2) == 0 (a thread's channel pair, and so its 18 weights,
-// stays fixed); other channel counts use the sliding form below.
+// item is independent, so all four waves stay busy.  Channel pairs run
+// fastest within a wave, so each tap is one contiguous 64-lane read.  The
+// pixels are taken in two passes:
+//  * interior pixels (all taps inside the image, (23 - 2 DIL)^2 of them): the
+//    nine taps sit at fixed offsets from the pixel, so an item costs its
+//    index, nine loads, the products and the GELUs;
+//  * border pixels: a tap outside the image reads `zero` (a zeroed LDS slot)
+//    instead, chosen per tap by a select, so the nine loads of an item still
+//    issue back to back.
+// Both passes sum the taps in the same order.  Needs 256 % (C/2) == 0 (a
+// thread's channel pair, and so its 18 weights, stays fixed); other channel
+// counts use the sliding form below.
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
 __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, const float* w, const float* bias,
                                const TI* zero, const float2* gtab)
 {
     constexpr int CP = C / 2;
     static_assert(256 % CP == 0, "channel pair fixed per thread");
+    constexpr int NI = kN - 2 * DIL;                 // interior side
+    constexpr int NB = kNN - NI * NI;                // border pixels
+    constexpr int NTOP = DIL * kN;                   // pixels in the top (and in the bottom) DIL rows
     const int cp = threadIdx.x % CP;
     float w0[9], w1[9];
 #pragma unroll
     for (int k = 0; k < 9; k++) { w0[k] = w[(2 * cp) * 9 + k]; w1[k] = w[(2 * cp + 1) * 9 + k]; }
     const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
+    auto finish = [&](int p, const float* v0, const float* v1) {
+        float a0 = b0, a1 = b1;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            a0 += w0[k] * v0[k];
+            a1 += w1[k] * v1[k];
+        }
+        store_pair(out + p * CSO + 2 * cp, gelu_tab(a0, gtab), gelu_tab(a1, gtab));
+    };
 #pragma unroll 2
-    for (int item = threadIdx.x; item < CP * kNN; item += 256) {
-        const int p = item / CP;
+    for (int item = threadIdx.x; item < CP * NI * NI; item += 256) {
+        const int q = item / CP;
+        const int iy = q / NI;
+        const int p = q + iy * (kN - NI) + DIL * (kN + 1);  // (iy + DIL) kN + (q - iy NI) + DIL
+        const TI* c = in + p * CSI + 2 * cp;
+        float v0[9], v1[9];
+#pragma unroll
+        for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+            for (int kx = 0; kx < 3; kx++)
+                load_pair(c + ((ky - 1) * DIL * kN + (kx - 1) * DIL) * CSI, v0[3 * ky + kx], v1[3 * ky + kx]);
+        }
+        finish(p, v0, v1);
+    }
+    for (int item = threadIdx.x; item < CP * NB; item += 256) {
+        const int q = item / CP;
+        // top DIL rows, bottom DIL rows, then the DIL left and DIL right
+        // pixels of each middle row
+        int p;
+        if (q < NTOP) {
+            p = q;
+        } else if (q < 2 * NTOP) {
+            p = (kN - DIL) * kN + (q - NTOP);
+        } else {
+            const int r = q - 2 * NTOP, row = DIL + r / (2 * DIL), k = r - (row - DIL) * (2 * DIL);
+            p = row * kN + (k < DIL ? k : kN - 2 * DIL + k);
+        }
         const int y = p / kN, x = p - y * kN;
         const TI* c = in + p * CSI + 2 * cp;
         const bool rv[3] = {y >= DIL, true, y + DIL < kN};
@@ -157,13 +197,7 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
                 load_pair(s, v0[3 * ky + kx], v1[3 * ky + kx]);
             }
         }
-        float a0 = b0, a1 = b1;
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            a0 += w0[k] * v0[k];
-            a1 += w1[k] * v1[k];
-        }
-        store_pair(out + p * CSO + 2 * cp, gelu_tab(a0, gtab), gelu_tab(a1, gtab));
+        finish(p, v0, v1);
     }
 }
 
@@ -294,7 +328,7 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g = lane >> 4, r = lane & 15;
     half4_t bw[NT][KT], bl[NT][KT];
-    float bb[NT][4];
+    float4_t bb[NT];
 #pragma unroll
     for (int n = 0; n < NT; n++) {
 #pragma unroll
@@ -358,17 +392,16 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
         // transposed product (see pw_mfma): 4 consecutive channels of pixel p
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            // the lane's eight K slots form one K = 32 operand; the small cross
-            // terms run in a chain of their own, so the two chains interleave
+            // the lane's eight K slots form one K = 32 operand; one chain from
+            // the bias: the two small cross terms first, then Wh ah
             static_assert(KT == 2, "eight K slots per lane");
             const half8_t a8 = cat8(ah[0], ah[1]), l8 = cat8(al[0], al[1]), w8 = cat8(bw[n][0], bw[n][1]);
-            float4_t acl = mfma32(cat8(bl[n][0], bl[n][1]), a8, (float4_t){0.f, 0.f, 0.f, 0.f});
-            acl = mfma32(w8, l8, acl);
-            float4_t acc = mfma32(w8, a8, (float4_t){0.f, 0.f, 0.f, 0.f});
-            acc += acl;
+            float4_t acc = mfma32(cat8(bl[n][0], bl[n][1]), a8, bb[n]);
+            acc = mfma32(w8, l8, acc);
+            acc = mfma32(w8, a8, acc);
             half4_t o;
 #pragma unroll
-            for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i] + bb[n][i], gtab);
+            for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i], gtab);
             *(half4_t*)(out + p * CSO + n * 16 + 4 * g) = o;
         }
     }
@@ -385,7 +418,7 @@ struct FaceSmem {
     alignas(16) half_t X[kPix * CS];  // 16-byte aligned: read as half4 / float2
     alignas(16) half_t Y[kPix * CS];
     alignas(16) float zero[4];  // read by out-of-image depthwise taps; zeroed at kernel start
-    alignas(16) float2 gtab[kGeluTab];  // GELU table, copied from the network's device copy at kernel start
+    alignas(16) float2 gtab[kGeluTabEntries];  // GELU table, copied from the network's device copy at kernel start
     VoxDesc desc;
     float red[8];
     int ibuf[4];
@@ -458,7 +491,7 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     if (MODE == 2 && !(face >= 2 && face <= 5)) return;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
     if (threadIdx.x < 4) s.zero[threadIdx.x] = 0.f;
-    for (int i = threadIdx.x; i < kGeluTab; i += 256) s.gtab[i] = w.gtab[i];
+    for (int i = threadIdx.x; i < kGeluTabEntries; i += 256) s.gtab[i] = w.gtab[i];
     __syncthreads();
     if constexpr (MODE == 2) face_solver_plus(s, w, face, axis); else face_solver<MODE>(s, w, face, axis);
     __syncthreads();

@@ -153,9 +153,35 @@ std::vector<half_t> pack_frag_input(const std::vector<float>& W, int N)
     return out;
 }
 
-// Table of gelu_tab: (Phi(x_i), Phi(x_i+1) - Phi(x_i)) at the nodes
-// x_i = -6 + i h of [-6, 6], Phi(x) = erfc(-x / sqrt 2) / 2 in double.
-std::vector<float2> gelu_table(int n = kGeluTab, double lo = kGeluLo, double inv_h = kGeluInvH)
+double phi_exact(double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); }
+
+// Table of gelu_tab: entry 0 = (0, 0) below the range, entry n + 1 = (1, 0)
+// above it, entry i + 1 = the line a + b x of Phi on interval i with slope b
+// of the chord and a shifted so that the largest deviations of Phi above and
+// below the line on the interval (sampled at 257 points) are equal.
+std::vector<float2> phi_line_table(int n = kGeluTab, double lo = kGeluLo, double inv_h = kGeluInvH)
+{
+    std::vector<float2> t(n + 2);
+    t[0] = make_float2(0.f, 0.f);
+    t[n + 1] = make_float2(1.f, 0.f);
+    const double h = 1.0 / inv_h;
+    for (int i = 0; i < n; i++) {
+        const double x0 = lo + (double)i * h, x1 = x0 + h;
+        const double b = (phi_exact(x1) - phi_exact(x0)) / h, a = phi_exact(x0) - b * x0;
+        double dmax = -1e300, dmin = 1e300;
+        for (int k = 0; k <= 256; k++) {
+            const double x = x0 + h * k / 256.0, d = phi_exact(x) - (a + b * x);
+            dmax = std::max(dmax, d);
+            dmin = std::min(dmin, d);
+        }
+        t[i + 1] = make_float2((float)(a + 0.5 * (dmax + dmin)), (float)b);
+    }
+    return t;
+}
+
+// Table of gelu_tab_sel: (Phi(x_i), Phi(x_i+1) - Phi(x_i)) at the nodes
+// x_i = lo + i h, Phi(x) = erfc(-x / sqrt 2) / 2 in double.
+std::vector<float2> gelu_table(int n, double lo, double inv_h)
 {
     std::vector<float2> t(n);
     auto phi = [](double x) { return 0.5 * std::erfc(-x / std::sqrt(2.0)); };
@@ -228,7 +254,7 @@ int build_face(const TensorMap& m, const FaceArch& A, Arena& ar, FaceNetDev& d)
     }
     if ((rc = get(m, pre + "head.weight", (size_t)A.clast, &t))) return rc;
     ar.add(pad(t->data, 4), &d.head_w);
-    ar.add(gelu_table(), &d.gtab);
+    ar.add(phi_line_table(), &d.gtab);
     if ((rc = get(m, pre + "head.bias", 1, &t))) return rc;
     d.head_b = t->data[0];
     return 0;
