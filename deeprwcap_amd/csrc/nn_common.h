@@ -30,8 +30,9 @@ struct VoxDesc {
     float lz[kN];          // layer value at each z (normalised)
     float cv[kMaxCuboids]; // cuboid values (normalised)
     float maxv;            // per-query max eps (dnn_solver.cpp:274-282)
-    uint32_t lzs[kN];            // lz and cv split into fp16 pairs hi | lo << 16,
-    uint32_t cvs[kMaxCuboids];   // hi + lo = the fp32 value to ~2^-22 relative
+    uint32_t lzs[kN];               // lz and cv split into fp16 pairs hi | lo << 16,
+    uint32_t cvs[kMaxCuboids + 1];  // hi + lo = the fp32 value to ~2^-22 relative; cvs[15] = 0 is
+                                    // read by lookups with no cuboid hit (mask | 1 << 15 scans to 15)
 };
 
 // fp16 pair (hi | lo << 16) with hi = fp16(v), lo = fp16(v - hi): an MFMA
@@ -86,6 +87,7 @@ __device__ inline void vox_desc_build(VoxDesc& d, const float* layers, const flo
         d.cv[lane] = cv;
         d.cvs[lane] = half_split(cv);
     }
+    if (lane == kMaxCuboids) d.cvs[lane] = 0u;
     if (lane == 0) d.maxv = m;
 }
 

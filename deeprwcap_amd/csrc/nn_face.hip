@@ -14,6 +14,8 @@
 // else 16x16x16; fp32 accumulate); depthwise 3x3, the small 1x1 layers,
 // batch-norm (folded into weights/bias on the host) and GELU (LDS table) run
 // on the VALU in fp32.
+#include <type_traits>
+
 #include "common.h"
 #include "nn_common.h"
 #include "nn_face.h"
@@ -126,7 +128,8 @@ __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float
 // pixels are taken in two passes:
 //  * interior pixels (all taps inside the image, (23 - 2 DIL)^2 of them): the
 //    nine taps sit at fixed offsets from the pixel, so an item costs its
-//    index, nine loads, the products and the GELUs;
+//    index, nine loads, the products and the GELUs; at dilation 1 an item is
+//    three adjacent pixels of a row (the 21 interior columns make 7 groups);
 //  * border pixels: a tap outside the image reads `zero` (a zeroed LDS slot)
 //    instead, chosen per tap by a select, so the nine loads of an item still
 //    issue back to back.
@@ -156,20 +159,51 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
         }
         store_pair(out + p * CSO + 2 * cp, gelu_tab(a0, gtab), gelu_tab(a1, gtab));
     };
-#pragma unroll 2
-    for (int item = threadIdx.x; item < CP * NI * NI; item += 256) {
-        const int q = item / CP;
-        const int iy = q / NI;
-        const int p = q + iy * (kN - NI) + DIL * (kN + 1);  // (iy + DIL) kN + (q - iy NI) + DIL
-        const TI* c = in + p * CSI + 2 * cp;
-        float v0[9], v1[9];
+    if constexpr (DIL == 1 && NI % 3 == 0) {
+        // three adjacent pixels per item: their taps are the 3 x 5 window
+        // around them (15 loads instead of 27), the index costs once per three
+        constexpr int G = 3, NG = NI / G;
+        for (int item = threadIdx.x; item < CP * NI * NG; item += 256) {
+            const int q = item / CP;
+            const int iy = q / NG;
+            const int p = (iy + 1) * kN + 1 + G * (q - iy * NG);  // first pixel of the group
+            const TI* c = in + (p - kN - 1) * CSI + 2 * cp;      // top-left tap of the window
+            float u0[3][G + 2], u1[3][G + 2];
 #pragma unroll
-        for (int ky = 0; ky < 3; ky++) {
+            for (int ky = 0; ky < 3; ky++) {
 #pragma unroll
-            for (int kx = 0; kx < 3; kx++)
-                load_pair(c + ((ky - 1) * DIL * kN + (kx - 1) * DIL) * CSI, v0[3 * ky + kx], v1[3 * ky + kx]);
+                for (int kx = 0; kx < G + 2; kx++) load_pair(c + (ky * kN + kx) * CSI, u0[ky][kx], u1[ky][kx]);
+            }
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                float v0[9], v1[9];
+#pragma unroll
+                for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+                    for (int kx = 0; kx < 3; kx++) {
+                        v0[3 * ky + kx] = u0[ky][j + kx];
+                        v1[3 * ky + kx] = u1[ky][j + kx];
+                    }
+                }
+                finish(p + j, v0, v1);
+            }
         }
-        finish(p, v0, v1);
+    } else {
+#pragma unroll 2
+        for (int item = threadIdx.x; item < CP * NI * NI; item += 256) {
+            const int q = item / CP;
+            const int iy = q / NI;
+            const int p = q + iy * (kN - NI) + DIL * (kN + 1);  // (iy + DIL) kN + (q - iy NI) + DIL
+            const TI* c = in + p * CSI + 2 * cp;
+            float v0[9], v1[9];
+#pragma unroll
+            for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++)
+                    load_pair(c + ((ky - 1) * DIL * kN + (kx - 1) * DIL) * CSI, v0[3 * ky + kx], v1[3 * ky + kx]);
+            }
+            finish(p, v0, v1);
+        }
     }
     for (int item = threadIdx.x; item < CP * NB; item += 256) {
         const int q = item / CP;
@@ -358,53 +392,66 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
         vmask[q] = vm[k];
         vlz[q] = d.lzs[k];
     }
-    for (int m = wave; m < kPix / 16; m += 4) {
-        const int p = m * 16 + r;
-        const int pc = min(p, kNN - 1);  // rows past the last pixel are zeroed below
-        const int h = pc / kN, w = pc - h * kN;
-        // fixed-axis coordinates (the moving axis' value is in range and masked out)
-        const int zs = oz + hz * h + wz * w, ys = oy + hy * h + wy * w, xs = ox + hx * h + wx * w;
-        const uint32_t fixedm = (zmov ? 0xffffu : (uint32_t)d.bz[zs]) & (ymov ? 0xffffu : (uint32_t)d.by[ys]) &
-                                (xmov ? 0xffffu : (uint32_t)d.bx[xs]);
-        const uint32_t lzp = d.lzs[zs];
-        const bool live = p < kNN;
-        uint32_t v[8];  // fp16 pairs hi | lo << 16
+    // Rows past the last pixel (p >= 529 in the last tile) repeat pixel 528's
+    // inputs: their outputs are finite and no later layer reads them (the
+    // depthwise taps stay inside the image, the tail and the head loop over
+    // p < 529).  The loop is instantiated for a moving z axis (layer value per
+    // slice) and a fixed one (layer value per pixel).
+    auto tiles = [&](auto zmov_c) {
+        constexpr bool ZM = decltype(zmov_c)::value;
+        for (int m = wave; m < kPix / 16; m += 4) {
+            const int p = m * 16 + r;
+            const int pc = min(p, kNN - 1);
+            const int h = pc / kN, w = pc - h * kN;
+            // fixed-axis coordinates (the moving axis' value is in range and masked out)
+            const int zs = oz + hz * h + wz * w, ys = oy + hy * h + wy * w, xs = ox + hx * h + wx * w;
+            const uint32_t fixedm = (ZM ? 0xffffu : (uint32_t)d.bz[zs]) & (ymov ? 0xffffu : (uint32_t)d.by[ys]) &
+                                    (xmov ? 0xffffu : (uint32_t)d.bx[xs]);
+            const uint32_t lzp = ZM ? 0u : d.lzs[zs];
+            uint32_t v[8];  // fp16 pairs hi | lo << 16
+            // Branch-free: the six cuboid reads always issue (slot 15 when no
+            // cuboid holds the voxel) and the empty asm, which takes all six
+            // values, keeps the compiler from moving each read into a branch.
+            uint32_t mm[6], cvv[6];
 #pragma unroll
-        for (int q = 0; q < 6; q++) {
-            const uint32_t mm = fixedm & vmask[q];
-            v[q] = mm ? d.cvs[__builtin_ctz(mm)] : (zmov ? vlz[q] : lzp);
+            for (int q = 0; q < 6; q++) {
+                mm[q] = fixedm & vmask[q];
+                cvv[q] = d.cvs[__builtin_ctz(mm[q] | (1u << kMaxCuboids))];
+            }
+            asm volatile("" : "+v"(cvv[0]), "+v"(cvv[1]), "+v"(cvv[2]), "+v"(cvv[3]), "+v"(cvv[4]), "+v"(cvv[5]));
+#pragma unroll
+            for (int q = 0; q < 6; q++) v[q] = mm[q] ? cvv[q] : (ZM ? vlz[q] : lzp);
+            // position planes (PosEmbed2D: linspace(0, 1, 23) over w, then h)
+            v[5] = g == 3 ? half_split((float)w * (1.0f / (kN - 1))) : v[5];
+            v[6] = g == 0 ? half_split((float)h * (1.0f / (kN - 1))) : 0u;
+            v[7] = 0u;
+            half4_t ah[KT], al[KT];
+#pragma unroll
+            for (int kt = 0; kt < KT; kt++) {
+                const uint32_t* e = v + 4 * kt;
+                const uint2 hi2 = make_uint2((e[0] & 0xffffu) | (e[1] << 16), (e[2] & 0xffffu) | (e[3] << 16));
+                const uint2 lo2 = make_uint2((e[0] >> 16) | (e[1] & 0xffff0000u), (e[2] >> 16) | (e[3] & 0xffff0000u));
+                ah[kt] = __builtin_bit_cast(half4_t, hi2);
+                al[kt] = __builtin_bit_cast(half4_t, lo2);
+            }
+            // transposed product (see pw_mfma): 4 consecutive channels of pixel p
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                // the lane's eight K slots form one K = 32 operand; one chain
+                // from the bias: the two small cross terms first, then Wh ah
+                static_assert(KT == 2, "eight K slots per lane");
+                const half8_t a8 = cat8(ah[0], ah[1]), l8 = cat8(al[0], al[1]), w8 = cat8(bw[n][0], bw[n][1]);
+                float4_t acc = mfma32(cat8(bl[n][0], bl[n][1]), a8, bb[n]);
+                acc = mfma32(w8, l8, acc);
+                acc = mfma32(w8, a8, acc);
+                half4_t o;
+#pragma unroll
+                for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i], gtab);
+                *(half4_t*)(out + p * CSO + n * 16 + 4 * g) = o;
+            }
         }
-        // position planes (PosEmbed2D: linspace(0, 1, 23) over w, then h)
-        v[5] = g == 3 ? half_split((float)w * (1.0f / (kN - 1))) : v[5];
-        v[6] = g == 0 ? half_split((float)h * (1.0f / (kN - 1))) : 0u;
-        v[7] = 0u;
-        half4_t ah[KT], al[KT];
-#pragma unroll
-        for (int kt = 0; kt < KT; kt++) {
-            uint32_t e[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) e[j] = live ? v[4 * kt + j] : 0u;
-            const uint2 hi2 = make_uint2((e[0] & 0xffffu) | (e[1] << 16), (e[2] & 0xffffu) | (e[3] << 16));
-            const uint2 lo2 = make_uint2((e[0] >> 16) | (e[1] & 0xffff0000u), (e[2] >> 16) | (e[3] & 0xffff0000u));
-            ah[kt] = __builtin_bit_cast(half4_t, hi2);
-            al[kt] = __builtin_bit_cast(half4_t, lo2);
-        }
-        // transposed product (see pw_mfma): 4 consecutive channels of pixel p
-#pragma unroll
-        for (int n = 0; n < NT; n++) {
-            // the lane's eight K slots form one K = 32 operand; one chain from
-            // the bias: the two small cross terms first, then Wh ah
-            static_assert(KT == 2, "eight K slots per lane");
-            const half8_t a8 = cat8(ah[0], ah[1]), l8 = cat8(al[0], al[1]), w8 = cat8(bw[n][0], bw[n][1]);
-            float4_t acc = mfma32(cat8(bl[n][0], bl[n][1]), a8, bb[n]);
-            acc = mfma32(w8, l8, acc);
-            acc = mfma32(w8, a8, acc);
-            half4_t o;
-#pragma unroll
-            for (int i = 0; i < 4; i++) o[i] = (half_t)gelu_tab(acc[i], gtab);
-            *(half4_t*)(out + p * CSO + n * 16 + 4 * g) = o;
-        }
-    }
+    };
+    if (zmov) tiles(std::true_type{}); else tiles(std::false_type{});
 }
 
 // ------------------------------------------------------------ architectures
