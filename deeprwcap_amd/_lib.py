@@ -31,6 +31,8 @@ SIGNATURES = {
     "drw_solver_set_seed": (_c_int, [_c_vp, _c_u64, _c_u64]),
     "drw_solver_get_counter": (_c_int, [_c_vp, ctypes.POINTER(_c_u64)]),
     "drw_solver_output_width": (_c_int, [_c_vp]),
+    "drw_solver_set_fdm_limits": (_c_int, [_c_vp, _c_int, _c_int]),
+    "drw_solver_failures": (_c_int, [_c_vp, ctypes.POINTER(_c_u64), ctypes.POINTER(_c_u64), ctypes.POINTER(_c_u64)]),
     "drw_solver_set_timing": (_c_int, [_c_vp, _c_int]),
     "drw_solver_stage_times": (_c_int, [_c_vp, _c_vp, _c_vp, _c_int]),
     "drw_solver_sample": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),

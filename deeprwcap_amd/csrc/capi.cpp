@@ -43,7 +43,7 @@ int drw_fdm_solve(int B, const int32_t* N, const int32_t* code, const float* eps
     a.G_off = G_off;
     a.order = order;
     a.tol = tol;
-    a.inner_floor = 1e-5f;
+    a.inner_floor = kFdmInnerFloor;
     a.max_inner = 2000;
     a.max_rounds = 12;
     a.iters = iters;
@@ -72,7 +72,7 @@ int drw_sample_flat(int B, const int32_t* N, const int32_t* axis, const double* 
     clear_error();
     if (B < 0) return set_error(DRW_EINVAL, "drw_sample_flat: B < 0");
     if (!N || !G || !G_off || !out) return set_error(DRW_EINVAL, "drw_sample_flat: null pointer");
-    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out, nullptr, nullptr};
+    FlatSampleArgs a{B, N, axis, G, G_off, center_eps, seed, counter, out, nullptr, nullptr, nullptr};
     return sample_flat_launch(a, (hipStream_t)stream);
 }
 
@@ -94,9 +94,13 @@ int drw_solver_create(int kind, int device_idx, drw_solver** out)
     s->kind = kind;
     s->device = device_idx;
     hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&s->stats, 4 * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemset(s->stats, 0, 4 * sizeof(uint64_t));
     if (e != hipSuccess) {
+        if (s->stats) (void)hipFree(s->stats);
+        if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
-        return set_error(DRW_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+        return set_error(DRW_EHIP, "drw_solver_create: %s", hipGetErrorString(e));
     }
     *out = s;
     return 0;
@@ -110,6 +114,10 @@ void drw_solver_destroy(drw_solver* s)
     s->layers.release(); s->cuboids.release(); s->out.release(); s->eps.release(); s->cen.release();
     s->axis.release(); s->Nq.release(); s->code.release(); s->eps_off.release(); s->G_off.release();
     s->G.release(); s->scratch.release();
+    s->status.release(); s->list.release(); s->count.release(); s->bad.release();
+    s->fb_eps.release(); s->fb_G.release(); s->fb_y.release(); s->fb_list.release(); s->fb_count.release();
+    s->fb_ctr.release();
+    if (s->stats) (void)hipFree(s->stats);
     if (s->nets) net_weights_free(s->nets);
     s->timer.release();
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -127,6 +135,14 @@ int drw_solver_load_models(drw_solver* s, const char* models_txt)
     if (s->nets) net_weights_free(s->nets);
     s->nets = w;
     s->backend = DRW_BACKEND_NN;
+    // FDM fallback slots of the network path (fixed size: no allocation per call)
+    if ((rc = s->fb_eps.ensure((size_t)kFallbackSlots * kMaxCells)) ||
+        (rc = s->fb_G.ensure((size_t)kFallbackSlots * 6 * kNetN * kNetN)) ||
+        (rc = s->fb_y.ensure((size_t)kFallbackSlots * kMaxCells)) || (rc = s->fb_list.ensure(kFallbackCap)) ||
+        (rc = s->fb_count.ensure(1)) || (rc = s->fb_ctr.ensure(2)))
+        return rc;
+    DRW_HIP(hipMemset(s->fb_ctr.p, 0, sizeof(uint32_t) * s->fb_ctr.n));
+    DRW_HIP(hipDeviceSynchronize());
     return 0;
 }
 
@@ -206,19 +222,38 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         StageTimer* tm = &s->timer;
         VoxelArgs va{B, L, C, nullptr, N, 0, s->eps.p, nullptr, nullptr};
         if ((rc = tm->begin(3, st)) || (rc = voxelize_launch(va, st)) || (rc = tm->end(st))) return rc;
-        if ((rc = fill_meta_launch(B, N, grad ? ax : nullptr, s->Nq.p, s->code.p, s->eps_off.p, s->G_off.p, st)))
+        if ((rc = s->status.ensure(B)) || (rc = s->list.ensure(B)) || (rc = s->count.ensure(1)) ||
+            (rc = s->bad.ensure(B)))
+            return rc;
+        if ((rc = fill_meta_launch(B, N, grad ? ax : nullptr, s->Nq.p, s->code.p, s->eps_off.p, s->G_off.p, s->bad.p,
+                                   s->count.p, st)))
             return rc;
         Workspace* ws = nullptr;
         if ((rc = get_workspace(st, &ws))) return rc;
         FdmBatch a{};
         a.B = B; a.N = s->Nq.p; a.code = s->code.p; a.eps = s->eps.p; a.eps_off = s->eps_off.p;
-        a.G = s->G.p; a.G_off = s->G_off.p; a.order = nullptr; a.tol = s->fdm_tol; a.inner_floor = 1e-5f;
-        a.max_inner = 2000; a.max_rounds = 12; a.iters = nullptr; a.relres = nullptr;
+        a.G = s->G.p; a.G_off = s->G_off.p; a.order = nullptr; a.tol = s->fdm_tol; a.inner_floor = kFdmInnerFloor;
+        a.max_inner = s->fdm_max_inner; a.max_rounds = s->fdm_max_rounds; a.iters = nullptr; a.relres = nullptr;
+        a.status = s->status.p;
         a.y_ws = ws->y_ws; a.s_ws = ws->s_ws; a.counters = ws->counters; a.n_hint = N;
-        if ((rc = tm->begin(4, st)) || (rc = fdm_launch(a, ws->fdm_grid, st)) || (rc = tm->end(st))) return rc;
+        // Acceptance as in ggft_solver::solve (ggft_solver.cpp:565-626): cubes that did
+        // not converge, or whose kernel is not finite (or, for GFT, does not sum to 1
+        // within 0.01 or has a negative panel) are solved again in FP64 mode; those
+        // that still fail get NaN hops.  The re-solve launch takes its count from the
+        // device and exits at once when the list is empty.
+        FdmBatch r = a;
+        r.order = s->list.p; r.B_dev = s->count.p; r.tol = a.tol < 1e-12 ? a.tol : 1e-12;
+        r.max_rounds = 40; r.max_inner = 4000;
+        FdmAcceptArgs acc1{B, s->Nq.p, s->code.p, s->G.p, s->G_off.p, s->status.p, nullptr, nullptr, s->list.p,
+                           s->count.p, nullptr, nullptr};
+        FdmAcceptArgs acc2{B, s->Nq.p, s->code.p, s->G.p, s->G_off.p, s->status.p, s->list.p, s->count.p, nullptr,
+                           nullptr, s->bad.p, s->stats};
+        if ((rc = tm->begin(4, st)) || (rc = fdm_launch(a, ws->fdm_grid, st)) || (rc = fdm_accept_launch(acc1, st)) ||
+            (rc = fdm_launch(r, ws->fdm_grid, st)) || (rc = fdm_accept_launch(acc2, st)) || (rc = tm->end(st)))
+            return rc;
         if (grad && (rc = center_eps_launch(B, s->Nq.p, ax, s->eps.p, s->eps_off.p, s->cen.p, st))) return rc;
         FlatSampleArgs fa{B, s->Nq.p, grad ? ax : nullptr, s->G.p, s->G_off.p, grad ? s->cen.p : nullptr,
-                          s->seed, s->counter, out, dg.qid, dg.ctr_dev};
+                          s->seed, s->counter, out, dg.qid, dg.ctr_dev, s->bad.p};
         if ((rc = tm->begin(5, st)) || (rc = sample_flat_launch(fa, st)) || (rc = tm->end(st))) return rc;
         s->counter += (uint64_t)B;
         return 0;
@@ -231,11 +266,38 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         q.ctr_dev = dg.ctr_dev;
         q.seed = s->seed; q.counter = s->counter; q.timer = &s->timer;
         int rc = query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, st);
+        // the FDM fallback of this batch, always enqueued: it reads the count of
+        // failed rows on the device
+        if (!rc) rc = nn_fallback_enqueue(s, q, st, &s->timer);
         if (rc) return rc;
         s->counter += (uint64_t)B;
         return 0;
     }
     return set_error(DRW_ESTATE, "solver has no backend: call drw_solver_load_models or drw_solver_use_fdm");
+}
+
+int drw_solver_set_fdm_limits(drw_solver* s, int max_rounds, int max_inner)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "set_fdm_limits: null solver");
+    if (max_rounds < 1 || max_inner < 1) return set_error(DRW_EINVAL, "set_fdm_limits: limits must be >= 1");
+    s->fdm_max_rounds = max_rounds;
+    s->fdm_max_inner = max_inner;
+    return 0;
+}
+
+int drw_solver_failures(drw_solver* s, uint64_t* fdm_resolved, uint64_t* failed, uint64_t* nn_fallbacks)
+{
+    clear_error();
+    if (!s) return set_error(DRW_EINVAL, "failures: null solver");
+    DRW_HIP(hipSetDevice(s->device));
+    DRW_HIP(hipStreamSynchronize(s->stream));
+    uint64_t h[4] = {};
+    DRW_HIP(hipMemcpy(h, s->stats, sizeof h, hipMemcpyDeviceToHost));
+    if (fdm_resolved) *fdm_resolved = h[0];
+    if (failed) *failed = h[1];
+    if (nn_fallbacks) *nn_fallbacks = h[2];
+    return 0;
 }
 
 static void warn_axis_ignored()

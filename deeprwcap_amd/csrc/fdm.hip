@@ -32,6 +32,8 @@
 //    workgroups in the (optional) host-given order, e.g. largest N first.
 #include "common.h"
 #include "fdm.h"
+#include "sampling.h"
+#include "voxel_fn.h"
 
 namespace drw {
 
@@ -39,22 +41,29 @@ namespace drw {
 // 156 KB of LDS, one per CU; cubes up to N = 16 take 256 threads and 53 KB,
 // so three share a CU and each hides the others' barrier and LDS latency
 // (one cube per CU leaves most of it waiting at the block reductions).
-template <int T, int MAXN, int PER_CU>
+template <int T, int MAXN, int PER_CU, bool FUSED>
 struct FdmSharedT {
     static constexpr int kThreads = T, kWaves = T / 64, kMaxN = MAXN, kPerCU = PER_CU;
+    static constexpr bool kFusedCG = FUSED;  // Chronopoulos-Gear inner loop (else textbook CG)
     static constexpr int kGuard = MAXN * MAXN, kCells = MAXN * MAXN * MAXN;
     float p[kCells + 2 * kGuard + T];  // slack covers the unused tail slots
     float wz[kCells + kGuard];         // scaled z+ couplings, zero guard plane below
     float ap[kCells];                  // A p of the current iteration (own cells only)
     float red[2][kWaves];
+    float2 red2[kWaves];
     double redd[kWaves];
     double rhs_val[8];
     int rhs_idx[8];
     double enorm;
     int ticket;
 };
-using FdmLarge = FdmSharedT<1024, kMaxN, 1>;
-using FdmSmall = FdmSharedT<256, 16, 3>;  // registers capped so that three fit a CU
+// The large class keeps textbook CG: the Chronopoulos-Gear loop needs one
+// more vector in registers (12 VGPRs at N = 23), which spills at 1024 threads
+// per workgroup (N = 23 batch 16.1 ms against 12.5 ms); the small class has
+// the registers and gains from the saved barrier (N = 16 batch 3.35 ms
+// against 3.45 ms).
+using FdmLarge = FdmSharedT<1024, kMaxN, 1, false>;
+using FdmSmall = FdmSharedT<256, 16, 3, true>;  // registers capped so that three fit a CU
 
 typedef __attribute__((address_space(3))) float lds_float;
 
@@ -76,6 +85,25 @@ __device__ __forceinline__ float block_sum(float v, float* red)
     float s = 0.f;
 #pragma unroll
     for (int w = 0; w < W; w++) s += red[w];
+    return s;
+}
+
+// Two sums with one barrier (the fused reduction of the Chronopoulos-Gear
+// loop).  The table is rewritten only after the caller's next barrier.
+template <int W>
+__device__ __forceinline__ float2 block_sum2(float a, float b, float2* red)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_float2(a, b);
+    __syncthreads();
+    float2 s = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        const float2 t = red[w];
+        s.x += t.x;
+        s.y += t.y;
+    }
     return s;
 }
 
@@ -153,25 +181,38 @@ __device__ void fdm_rhs(const float* E, int code, int* idx, double* val)
     for (int q = 0; q < 4; q++) { idx[4 + q] = minus[q]; val[4 + q] = D(minus[q]) * (1. / dsum + 1. / d_minus) / h * 2; }
 }
 
+// One cube's solve: its permittivities, outputs, the y workspace of the
+// calling workgroup and the iteration limits.
+struct CubeJob {
+    const float* E;     // N^3 permittivities, x fastest
+    double* G;          // 6 N^2 outputs
+    double* Y;          // N^3 FP64 refinement vector (workspace)
+    int code;           // right-hand side
+    double tol;
+    float inner_floor;
+    int max_inner, max_rounds;
+    int32_t* iters;     // optional single outputs
+    float* relres;
+    int32_t* status;    // 0 converged, 1 not within max_rounds
+};
+
 template <int N, typename SH>
-__device__ void fdm_cube(const FdmBatch& a, SH& sh, int b)
+__device__ void fdm_cube(const CubeJob& a, SH& sh)
 {
     static_assert(N <= SH::kMaxN, "cube larger than the size class");
     constexpr int NN = N * N, NNN = N * NN;
     constexpr int T = SH::kThreads, W = SH::kWaves;
     constexpr int CPT = (NNN + T - 1) / T;
     const int tid = threadIdx.x;
-    const float* E = a.eps + a.eps_off[b];
-    // workspace slot of this workgroup (fdm_launch sizes the grids to fit)
-    double* Y = a.y_ws + (size_t)blockIdx.x * SH::kCells;
-    float* S = a.s_ws + (size_t)blockIdx.x * SH::kCells;
-    const int code = a.code[b];
+    const float* E = a.E;
+    double* Y = a.Y;
+    const int code = a.code;
     float* P = sh.p + NN;  // valid for indices [-NN, NNN + NN)
 
     // ---- stage eps (in the A p array), clear guards and y, build the source ----
     float* WZ = sh.wz + NN;  // valid for indices [-NN, NNN)
     float* EL = sh.ap;       // eps staging during set-up and residual passes
-    for (int c = tid; c < NNN; c += T) { EL[c] = E[c]; Y[c] = 0.0; }
+    for (int c = tid; c < NNN; c += T) EL[c] = E[c];  // y starts at 0: the first update writes it
     for (int g = tid; g < NN; g += T) { P[g - NN] = 0.f; P[NNN + g] = 0.f; WZ[g - NN] = 0.f; }
     if (tid == 0) {
         fdm_rhs<N>(E, code, sh.rhs_idx, sh.rhs_val);
@@ -182,20 +223,26 @@ __device__ void fdm_cube(const FdmBatch& a, SH& sh, int b)
     __syncthreads();
 
     // ---- Jacobi scaling s_c = d_c^-1/2, d_c = sum_n w_cn + 2 eps_c nb(c) -----
-#pragma unroll 1
-    for (int c = tid; c < NNN; c += T) {
-        const int i = c % N, j = (c / N) % N, k = c / NN;
-        const float ec = EL[c];
-        float d = 0.f;
-        d += i > 0 ? harm(ec, EL[c - 1]) : 2.0f * ec;
-        d += i < N - 1 ? harm(ec, EL[c + 1]) : 2.0f * ec;
-        d += j > 0 ? harm(ec, EL[c - N]) : 2.0f * ec;
-        d += j < N - 1 ? harm(ec, EL[c + N]) : 2.0f * ec;
-        d += k > 0 ? harm(ec, EL[c - NN]) : 2.0f * ec;
-        d += k < N - 1 ? harm(ec, EL[c + NN]) : 2.0f * ec;
-        const float sc = 1.0f / sqrtf(d);
-        P[c] = sc;
-        S[c] = sc;
+    // (own cells in registers for the whole solve, all cells in P for the
+    // coupling set-up below)
+    float sc[CPT];
+#pragma unroll
+    for (int q = 0; q < CPT; q++) {
+        const int c = tid + q * T;
+        sc[q] = 0.f;
+        if (q < CPT - 1 || c < NNN) {
+            const int i = c % N, j = (c / N) % N, k = c / NN;
+            const float ec = EL[c];
+            float d = 0.f;
+            d += i > 0 ? harm(ec, EL[c - 1]) : 2.0f * ec;
+            d += i < N - 1 ? harm(ec, EL[c + 1]) : 2.0f * ec;
+            d += j > 0 ? harm(ec, EL[c - N]) : 2.0f * ec;
+            d += j < N - 1 ? harm(ec, EL[c + N]) : 2.0f * ec;
+            d += k > 0 ? harm(ec, EL[c - NN]) : 2.0f * ec;
+            d += k < N - 1 ? harm(ec, EL[c + NN]) : 2.0f * ec;
+            sc[q] = 1.0f / sqrtf(d);
+            P[c] = sc[q];
+        }
     }
     __syncthreads();
 
@@ -223,6 +270,8 @@ __device__ void fdm_cube(const FdmBatch& a, SH& sh, int b)
     float* R32 = P;  // residual hand-off reuses the p array (p is rebuilt from it)
     double rnorm = enorm;
     int total_it = 0;
+    bool have_y = false;  // y = 0 until the first update writes it
+    bool converged = false;
     for (int round = 0; round < a.max_rounds; round++) {
         // ---- FP64 residual r = e - L y, handed to the inner solve as FP32 S r ----
         // (one pass per cell, not unrolled, through the A p array)
@@ -251,91 +300,165 @@ __device__ void fdm_cube(const FdmBatch& a, SH& sh, int b)
             for (int s = 0; s < 8; s++)
                 if (sh.rhs_idx[s] == c) rv += sh.rhs_val[s];
             loc += rv * rv;
-            R32[c] = (float)rv * S[c];
+            R32[c] = (float)rv;  // scaled by s_c where the inner solve reads it
         }
         rnorm = sqrt(block_sum_d<W>(loc, sh.redd));  // also orders the R32 writes
-        if (!(rnorm > a.tol * enorm)) break;      // converged (or enorm == 0)
+        if (!(rnorm > a.tol * enorm)) {           // converged (or enorm == 0)
+            converged = true;
+            break;
+        }
 
         // ---- FP32 CG on the scaled system, right-hand side S r / ||r|| -------
         const float inv = (float)(1.0 / rnorm);
-        float xh[CPT], rh[CPT];
-        float rloc = 0.f;
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int c = tid + q * T;
-            xh[q] = 0.f;
-            rh[q] = (q < CPT - 1 || c < NNN) ? R32[c] * inv : 0.f;
-            rloc += rh[q] * rh[q];
-        }
-        float rr = block_sum<W>(rloc, sh.red[1]);
         double tgt = 0.3 * a.tol * enorm / rnorm;
         if (tgt < (double)a.inner_floor) tgt = a.inner_floor;
-        const float thresh = (float)(tgt * tgt) * rr;
-#pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int c = tid + q * T;
-            if (q < CPT - 1 || c < NNN) P[c] = rh[q];
-        }
-        __syncthreads();
+        const float tgt2 = (float)(tgt * tgt);
+        float xh[CPT];
         // Per-thread LDS bases; element q*T + k of each is cell c = tid + q*T
         // shifted by k - N^2 (P, WZ) or by k (AP).
         lds_float* Pb = lds_base(sh.p + tid);
         lds_float* Zb = lds_base(sh.wz + tid);
         lds_float* Ab = lds_base(sh.ap + tid);
-        for (int it = 0; it < a.max_inner; it++) {
-            float pl = 0.f;
+        if constexpr (SH::kFusedCG) {
+            // Chronopoulos-Gear CG (one fused reduction per iteration): r lives
+            // in P (the SpMV operand), p in AP, s = A p and x in registers.
+            // Per iteration: w = A r; (r, r) and (w, r) in one reduction;
+            // beta = g / g_prev, alpha = g / (d - beta g / alpha_prev);
+            // p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s; one
+            // barrier makes the new r visible to the next SpMV.  Convergence
+            // is seen one SpMV later than in textbook CG (the reduction that
+            // measures r_k also runs the k-th SpMV).
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
                 const int c = tid + q * T;
-                const int o = q * T + NN;  // P index of cell c relative to Pb
-                if (q < CPT - 1 || c < NNN) {
-                    const float pc = Pb[o];
-                    float v = pc;
-                    v -= w[0][q] * Pb[o - 1];
-                    v -= w[1][q] * Pb[o + 1];
-                    v -= w[2][q] * Pb[o - N];
-                    v -= w[3][q] * Pb[o + N];
-                    v -= Zb[q * T] * Pb[o - NN];
-                    v -= Zb[q * T + NN] * Pb[o + NN];
-                    Ab[q * T] = v;
-                    pl += pc * v;
-                }
-            }
-            const float pq = block_sum<W>(pl, sh.red[0]);
-            const float alpha = rr / pq;
-            float rl = 0.f;
-#pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
-                if (q < CPT - 1 || c < NNN) {
-                    xh[q] += alpha * Pb[q * T + NN];
-                    rh[q] -= alpha * Ab[q * T];
-                    rl += rh[q] * rh[q];
-                }
-            }
-            const float rr_new = block_sum<W>(rl, sh.red[1]);
-            total_it++;
-            if (!(rr_new > thresh)) break;
-            const float beta = rr_new / rr;
-            rr = rr_new;
-#pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
-                if (q < CPT - 1 || c < NNN) Pb[q * T + NN] = rh[q] + beta * Pb[q * T + NN];
+                xh[q] = 0.f;
+                if (q < CPT - 1 || c < NNN) P[c] = R32[c] * sc[q] * inv;  // R32 aliases P: in place
             }
             __syncthreads();
+            float sv[CPT];
+            float g_prev = 1.f, a_prev = 1.f, thresh = 0.f;
+            for (int it = 0; it < a.max_inner; it++) {
+                float wv[CPT];
+                float gl = 0.f, dl = 0.f;
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = tid + q * T;
+                    const int o = q * T + NN;  // P index of cell c relative to Pb
+                    wv[q] = 0.f;
+                    if (q < CPT - 1 || c < NNN) {
+                        const float rc = Pb[o];
+                        float v = rc;
+                        v -= w[0][q] * Pb[o - 1];
+                        v -= w[1][q] * Pb[o + 1];
+                        v -= w[2][q] * Pb[o - N];
+                        v -= w[3][q] * Pb[o + N];
+                        v -= Zb[q * T] * Pb[o - NN];
+                        v -= Zb[q * T + NN] * Pb[o + NN];
+                        wv[q] = v;
+                        gl += rc * rc;
+                        dl += v * rc;
+                    }
+                }
+                const float2 gd = block_sum2<W>(gl, dl, sh.red2);
+                const float g = gd.x;
+                if (it == 0) thresh = tgt2 * g;
+                else if (!(g > thresh)) break;
+                const float beta = it == 0 ? 0.f : g / g_prev;
+                const float den = gd.y - beta * g / a_prev;  // (p, A p) in exact arithmetic
+                if (!(den > 0.f)) break;                     // r is zero to rounding
+                const float alpha = g / den;
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = tid + q * T;
+                    const int o = q * T + NN;
+                    if (q < CPT - 1 || c < NNN) {
+                        const float rc = Pb[o];
+                        const float pn = it == 0 ? rc : rc + beta * Ab[q * T];
+                        sv[q] = it == 0 ? wv[q] : wv[q] + beta * sv[q];
+                        Ab[q * T] = pn;
+                        xh[q] += alpha * pn;
+                        Pb[o] = rc - alpha * sv[q];
+                    }
+                }
+                g_prev = g;
+                a_prev = alpha;
+                total_it++;
+                __syncthreads();
+            }
+        } else {
+            float rh[CPT];
+            float rloc = 0.f;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                const int c = tid + q * T;
+                xh[q] = 0.f;
+                rh[q] = (q < CPT - 1 || c < NNN) ? R32[c] * sc[q] * inv : 0.f;
+                rloc += rh[q] * rh[q];
+            }
+            float rr = block_sum<W>(rloc, sh.red[1]);
+            const float thresh = tgt2 * rr;
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                const int c = tid + q * T;
+                if (q < CPT - 1 || c < NNN) P[c] = rh[q];
+            }
+            __syncthreads();
+            for (int it = 0; it < a.max_inner; it++) {
+                float pl = 0.f;
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = tid + q * T;
+                    const int o = q * T + NN;  // P index of cell c relative to Pb
+                    if (q < CPT - 1 || c < NNN) {
+                        const float pc = Pb[o];
+                        float v = pc;
+                        v -= w[0][q] * Pb[o - 1];
+                        v -= w[1][q] * Pb[o + 1];
+                        v -= w[2][q] * Pb[o - N];
+                        v -= w[3][q] * Pb[o + N];
+                        v -= Zb[q * T] * Pb[o - NN];
+                        v -= Zb[q * T + NN] * Pb[o + NN];
+                        Ab[q * T] = v;
+                        pl += pc * v;
+                    }
+                }
+                const float pq = block_sum<W>(pl, sh.red[0]);
+                const float alpha = rr / pq;
+                float rl = 0.f;
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = tid + q * T;
+                    if (q < CPT - 1 || c < NNN) {
+                        xh[q] += alpha * Pb[q * T + NN];
+                        rh[q] -= alpha * Ab[q * T];
+                        rl += rh[q] * rh[q];
+                    }
+                }
+                const float rr_new = block_sum<W>(rl, sh.red[1]);
+                total_it++;
+                if (!(rr_new > thresh)) break;
+                const float beta = rr_new / rr;
+                rr = rr_new;
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = tid + q * T;
+                    if (q < CPT - 1 || c < NNN) Pb[q * T + NN] = rh[q] + beta * Pb[q * T + NN];
+                }
+                __syncthreads();
+            }
         }
         // ---- y += ||r|| S x -----------------------------------------------------
 #pragma unroll
         for (int q = 0; q < CPT; q++) {
             const int c = tid + q * T;
-            if (q < CPT - 1 || c < NNN) Y[c] += (double)S[c] * (double)xh[q] * rnorm;
+            if (q < CPT - 1 || c < NNN) Y[c] = (have_y ? Y[c] : 0.0) + (double)sc[q] * (double)xh[q] * rnorm;
         }
+        have_y = true;
         __syncthreads();
     }
 
     // ---- G_b = 2 eps_c y_c, GGFT panel order (ggft_solver.h:65-77) ----------
-    double* G = a.G + a.G_off[b];
+    double* G = a.G;
     for (int pb = tid; pb < 6 * NN; pb += T) {
         const int f = pb / NN, t = pb - f * NN, u = t % N, v = t / N;
         int c;
@@ -347,13 +470,32 @@ __device__ void fdm_cube(const FdmBatch& a, SH& sh, int b)
         case 4: c = u * N + v * NN; break;
         default: c = (N - 1) + u * N + v * NN; break;
         }
-        G[pb] = 2.0 * (double)E[c] * Y[c];
+        G[pb] = have_y ? 2.0 * (double)E[c] * Y[c] : 0.0;
     }
     if (tid == 0) {
-        if (a.iters) a.iters[b] = total_it;
-        if (a.relres) a.relres[b] = enorm > 0 ? (float)(rnorm / enorm) : 0.f;
+        if (a.iters) *a.iters = total_it;
+        if (a.relres) *a.relres = enorm > 0 ? (float)(rnorm / enorm) : 0.f;
+        if (a.status) *a.status = converged ? 0 : 1;
     }
     __syncthreads();
+}
+
+// The job of cube b of a batch, solved by the calling workgroup.
+__device__ __forceinline__ CubeJob batch_job(const FdmBatch& a, int b, int cells)
+{
+    CubeJob j;
+    j.E = a.eps + a.eps_off[b];
+    j.G = a.G + a.G_off[b];
+    j.Y = a.y_ws + (size_t)blockIdx.x * cells;  // fdm_launch sizes the grids to fit the workspace
+    j.code = a.code[b];
+    j.tol = a.tol;
+    j.inner_floor = a.inner_floor;
+    j.max_inner = a.max_inner;
+    j.max_rounds = a.max_rounds;
+    j.iters = a.iters ? a.iters + b : nullptr;
+    j.relres = a.relres ? a.relres + b : nullptr;
+    j.status = a.status ? a.status + b : nullptr;
+    return j;
 }
 
 __host__ __device__ constexpr bool fdm_n_ok(int n) { return n == 8 || n == 9 || n == 16 || n == 21 || n == 23; }
@@ -361,30 +503,34 @@ __host__ __device__ constexpr bool fdm_n_ok(int n) { return n == 8 || n == 9 || 
 // One persistent kernel per size class: workgroups take cubes by atomic
 // ticket (ctr[0]) and solve those of their class (supported N in
 // (NLO, SH::kMaxN]), skipping the rest; the last workgroup to leave resets
-// ctr for the next launch.  An unsupported N is marked (iters = -1) by
-// whichever class claims it.
+// ctr for the next launch.  An unsupported N is marked (iters = -1,
+// status = 2) by whichever class claims it.
 template <typename SH, int NLO>
 __global__ __launch_bounds__(SH::kThreads, SH::kPerCU) void fdm_pcg_kernel(FdmBatch a, uint32_t* ctr)
 {
     __shared__ SH sh;
+    const int nB = a.B_dev ? min(a.B, *a.B_dev) : a.B;  // the same value in every thread
     for (;;) {
         if (threadIdx.x == 0) sh.ticket = (int)atomicAdd(&ctr[0], 1u);
         __syncthreads();
         const int t = sh.ticket;
         __syncthreads();
-        if (t >= a.B) break;
+        if (t >= nB) break;
         const int b = a.order ? a.order[t] : t;
         const int n = a.N[b];
         const bool mine = (NLO == 0 || n > NLO) && !(n > SH::kMaxN && fdm_n_ok(n));
         if (!mine) continue;  // the other class (uniform across the workgroup)
         switch (n) {
-        case 8: if constexpr (NLO < 8) fdm_cube<8>(a, sh, b); break;
-        case 9: if constexpr (NLO < 9) fdm_cube<9>(a, sh, b); break;
-        case 16: if constexpr (NLO < 16) fdm_cube<16>(a, sh, b); break;
-        case 21: if constexpr (SH::kMaxN >= 21) fdm_cube<21>(a, sh, b); break;
-        case 23: if constexpr (SH::kMaxN >= 23) fdm_cube<23>(a, sh, b); break;
+        case 8: if constexpr (NLO < 8) fdm_cube<8>(batch_job(a, b, SH::kCells), sh); break;
+        case 9: if constexpr (NLO < 9) fdm_cube<9>(batch_job(a, b, SH::kCells), sh); break;
+        case 16: if constexpr (NLO < 16) fdm_cube<16>(batch_job(a, b, SH::kCells), sh); break;
+        case 21: if constexpr (SH::kMaxN >= 21) fdm_cube<21>(batch_job(a, b, SH::kCells), sh); break;
+        case 23: if constexpr (SH::kMaxN >= 23) fdm_cube<23>(batch_job(a, b, SH::kCells), sh); break;
         default:
-            if (threadIdx.x == 0 && a.iters) a.iters[b] = -1;
+            if (threadIdx.x == 0) {
+                if (a.iters) a.iters[b] = -1;
+                if (a.status) a.status[b] = 2;
+            }
             break;
         }
     }
@@ -429,6 +575,191 @@ int fdm_launch(const FdmBatch& a, int grid, hipStream_t stream)
                            a.counters + 2);
         DRW_HIP(hipGetLastError());
     }
+    return 0;
+}
+
+// ---------------------------------------------------------------- acceptance
+// Sum and minimum of one kernel by one wave (all lanes get them).
+__device__ __forceinline__ void wave_kernel_stats(const double* G, int M, double& sum, double& mn)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0.0, m = 1e300;
+    for (int i = lane; i < M; i += 64) {
+        const double g = G[i];
+        s += g;  // a NaN or infinity anywhere makes the sum non-finite
+        m = fmin(m, g);
+    }
+    sum = wave_sum(s);
+    mn = wave_min(m);
+}
+
+__device__ __forceinline__ bool kernel_ok(int status, int code, double sum, double mn)
+{
+    if (status != 0 || !isfinite(sum)) return false;
+    return code != kGFT || (fabs(sum - 1.0) <= 0.01 && mn >= 0.0);  // ggft_solver.cpp:599-625
+}
+
+// One wave per cube (first pass: all B; second pass: the listed cubes).
+__global__ __launch_bounds__(256) void fdm_accept_kernel(FdmAcceptArgs a)
+{
+    const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int n = a.list_in ? min(a.B, *a.count_in) : a.B;
+    if (a.list_in && a.stats && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)n);
+    if (w >= n) return;
+    const int b = a.list_in ? a.list_in[w] : w;
+    const int N = a.N[b];
+    double sum, mn;
+    wave_kernel_stats(a.G + a.G_off[b], 6 * N * N, sum, mn);
+    const bool ok = kernel_ok(a.status ? a.status[b] : 0, a.code[b], sum, mn);
+    if ((threadIdx.x & 63) == 0 && !ok) {
+        if (a.list_in) {
+            if (a.bad) a.bad[b] = 1;
+            if (a.stats) atomicAdd((unsigned long long*)&a.stats[1], 1ull);
+        } else {
+            a.list_out[atomicAdd(a.count_out, 1)] = b;
+        }
+    }
+}
+
+int fdm_accept_launch(const FdmAcceptArgs& a, hipStream_t stream)
+{
+    if (a.B <= 0) return 0;
+    hipLaunchKernelGGL(fdm_accept_kernel, dim3((a.B + 3) / 4), dim3(256), 0, stream, a);
+    DRW_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------- network fallback
+// One workgroup: the rows of out[B][width] that hold a NaN, in list.
+__global__ __launch_bounds__(1024) void nn_check_kernel(NnFallbackArgs a)
+{
+    __shared__ int n;
+    if (threadIdx.x == 0) n = 0;
+    __syncthreads();
+    for (int b = threadIdx.x; b < a.B; b += blockDim.x) {
+        bool nan = false;
+        for (int j = 0; j < a.width; j++) nan |= isnan(a.out[(size_t)b * a.width + j]);
+        if (nan) {
+            const int k = atomicAdd(&n, 1);
+            if (k < a.cap) a.list[k] = b;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        *a.count = min(n, a.cap);
+        if (n) {
+            atomicAdd((unsigned long long*)&a.stats[2], (unsigned long long)n);
+            if (n > a.cap) atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)(n - a.cap));
+        }
+    }
+}
+
+// Block-wide sum and minimum of one kernel (1024 threads), then acceptance.
+template <int W, typename SH>
+__device__ bool block_kernel_ok(const double* G, int M, int status, int code, SH& sh)
+{
+    double s = 0.0, m = 1e300;
+    for (int i = threadIdx.x; i < M; i += SH::kThreads) {
+        s += G[i];
+        m = fmin(m, G[i]);
+    }
+    const double sum = block_sum_d<W>(s, sh.redd);
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) sh.redd[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double mn = 1e300;
+#pragma unroll
+    for (int w = 0; w < W; w++) mn = fmin(mn, sh.redd[w]);
+    __syncthreads();
+    return kernel_ok(status, code, sum, mn);
+}
+
+// Workgroups take listed queries by ticket; each solves in its own slot of
+// the eps / G / y buffers: voxelize (raw permittivities, fill_structure
+// semantics), solve at tol, and on a failed acceptance once more in FP64
+// mode, then draw the hop with the query's own counter.  The last workgroup
+// to leave resets the ticket counter.
+__global__ __launch_bounds__(FdmLarge::kThreads, 1) void nn_fallback_kernel(NnFallbackArgs a)
+{
+    __shared__ FdmLarge sh;
+    __shared__ float Ly[kMaxLayers * 2], Cb[kMaxCuboids * 7];
+    constexpr int N = kMaxN, NN = N * N, NNN = N * NN;
+    const int n = *a.count;
+    const int slot = blockIdx.x;
+    float* E = a.eps + (size_t)slot * kMaxCells;
+    double* G = a.G + (size_t)slot * 6 * NN;
+    const bool grad = a.axis != nullptr;
+    for (;;) {
+        if (threadIdx.x == 0) sh.ticket = (int)atomicAdd(&a.ctr[0], 1u);
+        __syncthreads();
+        const int t = sh.ticket;
+        __syncthreads();
+        if (t >= n) break;
+        const int q = a.list[t];
+        if (threadIdx.x < kMaxLayers * 2) Ly[threadIdx.x] = a.layers[(size_t)q * kMaxLayers * 2 + threadIdx.x];
+        if (threadIdx.x < kMaxCuboids * 7) Cb[threadIdx.x] = a.cuboids[(size_t)q * kMaxCuboids * 7 + threadIdx.x];
+        __syncthreads();
+        float lv[kMaxLayers], cv[kMaxCuboids];
+        for (int l = 0; l < kMaxLayers; l++) lv[l] = Ly[2 * l + 1];
+        for (int c = 0; c < kMaxCuboids; c++) cv[c] = Cb[7 * c + 6];
+        for (int v = threadIdx.x; v < NNN; v += FdmLarge::kThreads) {
+            const int x = v % N, y = (v / N) % N, z = v / NN;
+            E[v] = voxel_value(Ly, lv, Cb, cv, (2.0f * x + 1.0f) / N - 1.0f, (2.0f * y + 1.0f) / N - 1.0f,
+                               (2.0f * z + 1.0f) / N - 1.0f);
+        }
+        __syncthreads();  // the slot cube is visible to the whole workgroup
+        const int ax = grad ? a.axis[q] : 2;
+        CubeJob j;
+        j.E = E;
+        j.G = G;
+        j.Y = a.y_ws + (size_t)slot * kMaxCells;
+        j.code = grad ? (ax == 0 ? kWVTX : ax == 1 ? kWVTY : kWVTZ) : kGFT;
+        j.tol = a.tol;
+        j.inner_floor = kFdmInnerFloor;
+        j.max_inner = 2000;
+        j.max_rounds = 12;
+        j.iters = nullptr;
+        j.relres = nullptr;
+        j.status = &sh.ticket;  // the ticket is read above; reused as the status cell
+        fdm_cube<N>(j, sh);
+        bool ok = block_kernel_ok<FdmLarge::kWaves>(G, 6 * NN, sh.ticket, j.code, sh);
+        if (!ok) {
+            j.tol = 1e-12;
+            j.max_rounds = 40;
+            j.max_inner = 4000;
+            fdm_cube<N>(j, sh);
+            ok = block_kernel_ok<FdmLarge::kWaves>(G, 6 * NN, sh.ticket, j.code, sh);
+            if (threadIdx.x == 0) {
+                atomicAdd((unsigned long long*)&a.stats[0], 1ull);
+                if (!ok) atomicAdd((unsigned long long*)&a.stats[1], 1ull);
+            }
+        }
+        if (threadIdx.x < 64) {
+            const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
+            // centre cell of the raw cube (odd N, ggft_solver.cpp:649-671)
+            flat_draw_wave(G, N, grad, E[(N / 2) * (1 + N + NN)], a.seed, a.qid ? a.qid[q] : base + (uint64_t)q, !ok,
+                           a.out + (size_t)q * a.width);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&a.ctr[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&a.ctr[0], 0u);
+            atomicExch(&a.ctr[1], 0u);
+        }
+    }
+}
+
+int nn_fallback_launch(const NnFallbackArgs& a, hipStream_t stream)
+{
+    if (a.B <= 0) return 0;
+    if (a.grid <= 0 || a.cap <= 0) return set_error(-2, "nn_fallback_launch: no fallback buffers");
+    hipLaunchKernelGGL(nn_check_kernel, dim3(1), dim3(1024), 0, stream, a);
+    DRW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(nn_fallback_kernel, dim3(a.grid), dim3(FdmLarge::kThreads), 0, stream, a);
+    DRW_HIP(hipGetLastError());
     return 0;
 }
 

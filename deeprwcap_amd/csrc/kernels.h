@@ -29,6 +29,7 @@ struct FlatSampleArgs {
     float* out;                // [B, 3] or [B, 5]
     const uint64_t* qid;       // [B] per-query counters (optional; else counter + b)
     const uint64_t* ctr_dev;   // device-resident counter base (optional; replaces counter)
+    const uint8_t* bad;        // [B] queries whose kernel failed acceptance: NaN results (optional)
 };
 int sample_flat_launch(const FlatSampleArgs& a, hipStream_t stream);
 
@@ -39,6 +40,8 @@ int center_eps_launch(int B, const int32_t* N, const int32_t* axis, const float*
 
 namespace drw {
 int advance_counter_launch(uint64_t* ctr, uint64_t n, hipStream_t stream);
+// Per-query solve metadata of a uniform-N batch; also zeroes bad[B] and the
+// acceptance list count (both optional).
 int fill_meta_launch(int B, int N, const int32_t* axis, int32_t* Nq, int32_t* code, int64_t* eps_off, int64_t* G_off,
-                     hipStream_t stream);
+                     uint8_t* bad, int32_t* count, hipStream_t stream);
 }  // namespace drw

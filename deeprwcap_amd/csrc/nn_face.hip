@@ -534,6 +534,11 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     const int b = a.full ? item / 6 : item;
     const int face = a.full ? item % 6 : a.face[b];
     const int axis = a.axis ? a.axis[b] : 2;
+    if (face < 0) {  // failed face draw: a NaN result (the Poisson or the Gradient1 kernel writes it)
+        const int width = MODE == 0 ? 3 : 5;
+        if (MODE != 2 && threadIdx.x < width) a.out[(size_t)b * width + threadIdx.x] = __builtin_nanf("");
+        return;
+    }
     if (MODE == 1 && !(face == 0 || face == 1)) return;
     if (MODE == 2 && !(face >= 2 && face <= 5)) return;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);

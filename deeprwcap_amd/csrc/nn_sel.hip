@@ -348,8 +348,9 @@ __global__ __launch_bounds__(256) void selector_head_kernel(SelHeadArgs a, SelNe
             uint32_t u[4];
             const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
             query_uniforms(a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, 0u, u);
-            const int f = small_cdf_draw(p, 6, u[0]);
-            a.face[b] = f < 0 ? 0 : f;
+            // -1: no drawable face (non-finite or zero probabilities); the face
+            // kernel then writes a NaN result, which the FDM fallback picks up
+            a.face[b] = small_cdf_draw(p, 6, u[0]);
         }
     }
 }

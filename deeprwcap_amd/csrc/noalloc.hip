@@ -101,7 +101,9 @@ extern "C" int drw_solver_sample_device_noalloc(drw_solver* s, int B, const floa
     q.counter = s->counter;
     q.timer = nullptr;  // stage events are created on demand: not here
     uint8_t* p = buf;
-    const int rc = query_nn_launch(s->nets, q, &p, &cap, stream ? (hipStream_t)stream : s->stream);
+    const hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+    int rc = query_nn_launch(s->nets, q, &p, &cap, st);
+    if (!rc) rc = nn_fallback_enqueue(s, q, st, nullptr);  // its buffers come with the models
     if (rc) return rc;
     s->counter += (uint64_t)B;
     return 0;

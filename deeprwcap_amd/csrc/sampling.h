@@ -47,17 +47,6 @@ __device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, int d)
     return ((uint64_t)hi << 32) | lo;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_max(T v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const T u = __shfl_xor(v, o, 64);
-        v = u > v ? u : v;
-    }
-    return v;
-}
-
 __device__ __forceinline__ uint32_t quant(float a, int s) { return (uint32_t)ldexpf(fabsf(a), s); }
 __device__ __forceinline__ uint32_t quant(double a, int s) { return (uint32_t)ldexp(fabs(a), s); }
 __device__ __forceinline__ int frexp_exp(float m) { int e; frexpf(m, &e); return e; }
@@ -105,15 +94,10 @@ __device__ int wave_cdf_draw(Ptr w, int M, uint32_t u)
     uint64_t base = 0;  // cumulative sum of the rows before j0
     for (int j0 = 0; j0 < M; j0 += 64) {
         const int i = j0 + lane;
-        uint32_t incl = i < M ? quant(w[i], s) : 0u;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const uint32_t o = __shfl_up((int)incl, d, 64);
-            if (lane >= d) incl += o;
-        }
+        const uint32_t incl = wave_inclusive_scan(i < M ? quant(w[i], s) : 0u);
         const uint64_t hit = __ballot(base + incl > r);
         if (hit) return j0 + __ffsll((unsigned long long)hit) - 1;
-        base += (uint32_t)__shfl((int)incl, 63, 64);
+        base += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
     }
     return M - 1;
 }
@@ -141,6 +125,49 @@ __host__ __device__ __forceinline__ int axis_face_remap(int f, int axis)
     if (axis == 0) return f == 0 ? 4 : f == 1 ? 5 : f == 4 ? 1 : f == 5 ? 0 : f;
     if (axis == 1) return f == 0 ? 2 : f == 1 ? 3 : f == 2 ? 1 : f == 3 ? 0 : f;
     return f;
+}
+
+// Flat inverse-CDF next hop over one query's 6 N^2 FDM kernel by one wave
+// (the sampling semantics of ggft_solver::postprocess, ggft_solver.cpp:
+// 638-685: normalise |G| and invert its CDF in GGFT panel order), with the
+// query's first Philox word.  All 64 lanes call.
+//   Poisson  (!grad): o = (x, y, z)
+//   gradient (grad):  o = (x, y, z, W sign(G_i), eps_centre), W = sum |G|
+// A failed query (bad, or no drawable bin) gets NaN in every output.
+__device__ inline void flat_draw_wave(const double* G, int N, bool grad, float center_eps, uint64_t seed,
+                                      uint64_t counter, bool bad, float* o)
+{
+    const int lane = threadIdx.x & 63;
+    const int NN = N * N, M = 6 * NN;
+    const int width = grad ? 5 : 3;
+    uint32_t u[4];
+    query_uniforms(seed, counter, 0u, u);
+    const int i = bad ? -1 : wave_cdf_draw<double>(G, M, u[0]);
+    if (i < 0) {
+        if (lane < width) o[lane] = __builtin_nanf("");
+        return;
+    }
+    const int f = i / NN, t = i - f * NN;
+    if (!grad) {
+        if (lane == 0) {
+            float p[3];
+            panel_point(N, t, f, p);
+            o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+        }
+        return;
+    }
+    double wsum = 0.0;
+    for (int k = lane; k < M; k += 64) wsum += fabs(G[k]);
+    wsum = wave_sum(wsum);
+    if (lane == 0) {
+        float p[3];
+        panel_point(N, t, f, p);
+        const double g = G[i];
+        const float sg = g > 0 ? 1.0f : (g < 0 ? -1.0f : 0.0f);
+        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+        o[3] = (float)wsum * sg;
+        o[4] = center_eps;
+    }
 }
 
 }  // namespace drw

@@ -8,6 +8,7 @@
 
 #include "../../include/dnnsolver_c.h"
 #include "common.h"
+#include "fdm.h"
 #include "query.h"
 #include "runtime.h"
 
@@ -59,4 +60,53 @@ struct drw_solver {
     drw::DevBuf<double> G;
     drw::DevBuf<uint8_t> scratch;
     drw::StageTimer timer;
+    // FDM path: first-solve limits, per-cube status, acceptance list, bad flags
+    int fdm_max_rounds = 12, fdm_max_inner = 2000;
+    drw::DevBuf<int32_t> status, list, count;
+    drw::DevBuf<uint8_t> bad;
+    // network path: FDM fallback slots (allocated with the models)
+    drw::DevBuf<float> fb_eps;
+    drw::DevBuf<double> fb_G, fb_y;
+    drw::DevBuf<int32_t> fb_list, fb_count;
+    drw::DevBuf<uint32_t> fb_ctr;
+    // [0] FDM re-solves, [1] failed queries, [2] network fallbacks
+    uint64_t* stats = nullptr;
 };
+
+namespace drw {
+constexpr int kFallbackSlots = 32;     // workgroups (and buffer slots) of the network-path fallback
+constexpr int kFallbackCap = 16384;    // failed network queries per batch that the fallback takes
+
+// Enqueues the network-path FDM fallback for the batch q just enqueued
+// (queries whose result holds a NaN are solved by the FDM and drawn with their
+// own counters); allocates nothing.
+inline int nn_fallback_enqueue(drw_solver* s, const QueryArgs& q, hipStream_t st, StageTimer* tm)
+{
+    if (!s->fb_ctr.p) return set_error(DRW_ESTATE, "network fallback buffers missing: load the models first");
+    NnFallbackArgs f{};
+    f.B = q.B;
+    f.width = s->kind == DRW_GRADIENT ? 5 : 3;
+    f.cap = kFallbackCap;
+    f.out = q.out;
+    f.layers = q.layers;
+    f.cuboids = q.cuboids;
+    f.axis = q.axis;
+    f.list = s->fb_list.p;
+    f.count = s->fb_count.p;
+    f.stats = s->stats;
+    f.seed = q.seed;
+    f.counter = q.counter;
+    f.qid = q.qid;
+    f.ctr_dev = q.ctr_dev;
+    f.tol = s->fdm_tol;
+    f.grid = kFallbackSlots;
+    f.eps = s->fb_eps.p;
+    f.G = s->fb_G.p;
+    f.y_ws = s->fb_y.p;
+    f.ctr = s->fb_ctr.p;
+    int rc;
+    if (tm && (rc = tm->begin(6, st))) return rc;
+    if ((rc = nn_fallback_launch(f, st))) return rc;
+    return tm ? tm->end(st) : 0;
+}
+}  // namespace drw

@@ -3,39 +3,9 @@
 #include "common.h"
 #include "kernels.h"
 #include "sampling.h"
+#include "voxel_fn.h"
 
 namespace drw {
-
-// Per-query maximum permittivity over all layer and cuboid entries (padded
-// entries are -9999 and never win), dnn_solver.cpp:274-282.
-__device__ __forceinline__ float query_max(const float* Ly, const float* Cb)
-{
-    float m = Ly[1];
-    for (int l = 1; l < kMaxLayers; l++) m = fmaxf(m, Ly[2 * l + 1]);
-    for (int c = 0; c < kMaxCuboids; c++) m = fmaxf(m, Cb[7 * c + 6]);
-    return m;
-}
-
-// fill_structure_kernel semantics (cuda_kernels.cu:21-84): the last layer
-// whose min_z <= z_centre (list ends at the first padded min_z), overridden by
-// the first cuboid whose closed box holds the voxel centre.
-__device__ __forceinline__ float voxel_value(const float* Ly, const float* lv, const float* Cb, const float* cv,
-                                             float xc, float yc, float zc)
-{
-    int li = 0;
-    for (int l = 0; l < kMaxLayers; l++) {
-        const float mz = Ly[2 * l];
-        if (mz == kPad) break;
-        if (zc >= mz) li = l; else break;
-    }
-    float v = lv[li];
-    for (int c = 0; c < kMaxCuboids; c++) {
-        const float* q = Cb + 7 * c;
-        if (q[0] == kPad) break;
-        if (xc >= q[0] && xc <= q[3] && yc >= q[1] && yc <= q[4] && zc >= q[2] && zc <= q[5]) { v = cv[c]; break; }
-    }
-    return v;
-}
 
 // One workgroup per query; the 135-float description is staged in LDS.
 __global__ __launch_bounds__(256) void voxelize_kernel(VoxelArgs a)
@@ -78,49 +48,15 @@ int voxelize_launch(const VoxelArgs& a, hipStream_t stream)
 }
 
 // Flat inverse-CDF next hop over the 6 N^2 FDM kernel of each query
-// (the sampling semantics of ggft_solver::postprocess, ggft_solver.cpp:638-685:
-// normalise |G| and invert its CDF in GGFT panel order).  One wave per query.
-//   Poisson  (axis < 0): out[b] = (x, y, z)
-//   gradient (axis >= 0): out[b] = (x, y, z, W sign(G_i), eps_centre), W = sum |G|
+// (flat_draw_wave), one wave per query.
 __global__ __launch_bounds__(256) void sample_flat_kernel(FlatSampleArgs a)
 {
     const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (b >= a.B) return;
-    const int lane = threadIdx.x & 63;
-    const int N = a.N[b], NN = N * N, M = 6 * NN;
-    const double* G = a.G + a.G_off[b];
-    uint32_t u[4];
     const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
-    query_uniforms(a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, 0u, u);
-    const int i = wave_cdf_draw<double>(G, M, u[0]);
     const bool grad = a.axis != nullptr;
-    const int width = grad ? 5 : 3;
-    float* o = a.out + (size_t)b * width;
-    if (i < 0) {
-        if (lane < width) o[lane] = __builtin_nanf("");
-        return;
-    }
-    const int f = i / NN, t = i - f * NN;
-    if (!grad) {
-        if (lane == 0) {
-            float p[3];
-            panel_point(N, t, f, p);
-            o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
-        }
-        return;
-    }
-    double wsum = 0.0;
-    for (int k = lane; k < M; k += 64) wsum += fabs(G[k]);
-    wsum = wave_sum(wsum);
-    if (lane == 0) {
-        float p[3];
-        panel_point(N, t, f, p);
-        const double g = G[i];
-        const float sg = g > 0 ? 1.0f : (g < 0 ? -1.0f : 0.0f);
-        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
-        o[3] = (float)wsum * sg;
-        o[4] = a.center_eps ? a.center_eps[b] : 0.0f;
-    }
+    flat_draw_wave(a.G + a.G_off[b], a.N[b], grad, grad && a.center_eps ? a.center_eps[b] : 0.0f, a.seed,
+                   a.qid ? a.qid[b] : base + (uint64_t)b, a.bad && a.bad[b], a.out + (size_t)b * (grad ? 5 : 3));
 }
 
 int sample_flat_launch(const FlatSampleArgs& a, hipStream_t stream)
@@ -170,10 +106,12 @@ namespace drw {
 // Per-query solve metadata for a uniform-N batch: resolution, right-hand
 // side (GFT, or the WVT code of the gradient axis) and packed offsets.
 __global__ void fill_meta_kernel(int B, int N, const int32_t* axis, int32_t* Nq, int32_t* code, int64_t* eps_off,
-                                 int64_t* G_off)
+                                 int64_t* G_off, uint8_t* bad, int32_t* count)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0 && count) *count = 0;
     if (b >= B) return;
+    if (bad) bad[b] = 0;
     Nq[b] = N;
     int c = kGFT;
     if (axis) {
@@ -186,11 +124,11 @@ __global__ void fill_meta_kernel(int B, int N, const int32_t* axis, int32_t* Nq,
 }
 
 int fill_meta_launch(int B, int N, const int32_t* axis, int32_t* Nq, int32_t* code, int64_t* eps_off, int64_t* G_off,
-                     hipStream_t stream)
+                     uint8_t* bad, int32_t* count, hipStream_t stream)
 {
     if (B <= 0) return 0;
     hipLaunchKernelGGL(fill_meta_kernel, dim3((B + 255) / 256), dim3(256), 0, stream, B, N, axis, Nq, code, eps_off,
-                       G_off);
+                       G_off, bad, count);
     DRW_HIP(hipGetLastError());
     return 0;
 }

@@ -100,6 +100,22 @@ int drw_solver_backend(const drw_solver* s);
 int drw_solver_set_seed(drw_solver* s, uint64_t seed, uint64_t counter);
 int drw_solver_get_counter(const drw_solver* s, uint64_t* counter);
 
+/* Failure handling (ggft_solver::solve, ggft/src/ggft_solver.cpp:565-626).
+ * FDM backend: a cube that did not converge within the first solve's limits,
+ * or whose kernel is not finite, or (GFT) does not sum to 1 within 0.01 or
+ * has a negative panel, is solved again in FP64 mode (tol 1e-12, where the
+ * reference falls back to SparseLU); if it still fails, its hop is NaN.
+ * Network backend: a query whose result holds a NaN (a failed draw: a
+ * non-finite or all-zero kernel or face distribution) is solved by the FDM at
+ * N = 23 with the same acceptance and drawn with its own Philox counter (no
+ * counterpart in the reference, which has no fallback for its networks).
+ * set_fdm_limits bounds the first FDM solve (refinement rounds, FP32
+ * iterations per round; defaults 12, 2000).  failures waits for the solver's
+ * stream and returns the running counts: FDM re-solves, queries left failed
+ * (NaN hops), network queries sent to the FDM fallback. */
+int drw_solver_set_fdm_limits(drw_solver* s, int max_rounds, int max_inner);
+int drw_solver_failures(drw_solver* s, uint64_t* fdm_resolved, uint64_t* failed, uint64_t* nn_fallbacks);
+
 /* Output floats per query: 3 (Poisson) or 5 (gradient). */
 int drw_solver_output_width(const drw_solver* s);
 
