@@ -367,6 +367,7 @@ struct drw_graph {
     hipGraphExec_t exec = nullptr;
     uint64_t* d_counter = nullptr;
     int B = 0;
+    hipEvent_t before = nullptr, after = nullptr;  // orders a caller's stream with the solver's
 };
 
 int drw_solver_graph_create(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
@@ -382,6 +383,8 @@ int drw_solver_graph_create(drw_solver* s, int B, const float* layers, const flo
     g->B = B;
     hipError_t e = hipMalloc(&g->d_counter, sizeof(uint64_t));
     if (e == hipSuccess) e = hipMemcpy(g->d_counter, &s->counter, sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->before, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->after, hipEventDisableTiming);
     if (e != hipSuccess) {
         delete g;
         return set_error(DRW_EHIP, "graph_create: %s", hipGetErrorString(e));
@@ -421,7 +424,21 @@ int drw_graph_launch(drw_graph* g, void* stream)
 {
     clear_error();
     if (!g || !g->exec) return set_error(DRW_EINVAL, "graph_launch: null graph");
-    DRW_HIP(hipGraphLaunch(g->exec, stream ? (hipStream_t)stream : g->s->stream));
+    // The captured step uses the solver's buffers and the workspace and ticket
+    // counters of the solver's stream, so it always replays on that stream.
+    // A caller's stream is ordered around it: the replay starts after the work
+    // queued on `stream` so far, and later work on `stream` waits for it.
+    hipStream_t own = g->s->stream;
+    hipStream_t st = stream ? (hipStream_t)stream : own;
+    if (st != own) {
+        DRW_HIP(hipEventRecord(g->before, st));
+        DRW_HIP(hipStreamWaitEvent(own, g->before, 0));
+    }
+    DRW_HIP(hipGraphLaunch(g->exec, own));
+    if (st != own) {
+        DRW_HIP(hipEventRecord(g->after, own));
+        DRW_HIP(hipStreamWaitEvent(st, g->after, 0));
+    }
     return 0;
 }
 
@@ -440,6 +457,8 @@ void drw_graph_destroy(drw_graph* g)
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->d_counter) (void)hipFree(g->d_counter);
+    if (g->before) (void)hipEventDestroy(g->before);
+    if (g->after) (void)hipEventDestroy(g->after);
     delete g;
 }
 

@@ -151,7 +151,12 @@ int drw_solver_sample_ids(drw_solver* s, int B, const float* layers, const float
  * submission that draws fresh uniforms: launch k uses query counters
  * c0 + k B with c0 the solver's counter at creation (the solver's own counter
  * is not advanced; drw_graph_counter reads the graph's).  The graph must be
- * destroyed before its solver. */
+ * destroyed before its solver.  A replay always runs on the solver's own
+ * stream (the captured step uses that stream's workspace and ticket
+ * counters); a `stream` argument other than that is ordered around it with
+ * events (the replay waits for the work queued on `stream`, later work on
+ * `stream` waits for the replay).  drw_graph_counter waits for the solver's
+ * stream.  A solver, and its graphs, serve one stream of work at a time. */
 typedef struct drw_graph drw_graph;
 int drw_solver_graph_create(drw_solver* s, int B, const float* layers, const float* cuboids, const int32_t* axis,
                             float* out, drw_graph** graph);

@@ -89,3 +89,45 @@ def test_submit_many_matches_single(models_txt):
     sv.close()
     for i, (k, s) in enumerate(zip(kinds, single)):
         np.testing.assert_array_equal(many[i, :len(s)], s, err_msg=f"request {i} {reqs[i][:3]}")
+
+
+@pytest.mark.parametrize("rate", [1e5, 1e6], ids=["1e5qps", "1e6qps"])
+def test_load_with_probes_alone_equals_mix(rate):
+    """Config-5 serving load: 64 host threads issue Poisson-arrival queries of
+    mixed N {16, 21, 23} and both kinds at the offered rate (shipped weights);
+    meanwhile probe queries with fixed counters are submitted into that mix.
+    The achieved rate follows the offered one, p50/p99 latency are reported,
+    and each probe's result in the mix equals its result alone."""
+    import threading
+    import time
+    from deeprwcap_amd import structures as S
+    from deeprwcap_amd.serving import Server
+    pool = 4096
+    rng = np.random.default_rng(11)
+    Ns = rng.choice(np.array([16, 21, 23], np.int32), pool)
+    L, C, _ = S.layered_boxes(pool, Ns, seed=21)
+    kinds = rng.integers(0, 2, pool).astype(np.int32)
+    axes = rng.integers(0, 3, pool).astype(np.int32)
+    probes = _requests(96, 31)
+    sv = Server(0, os.path.join(ROOT, "models", "models.txt"), max_batch=1024, max_inflight=4, hbm_budget_mb=512)
+    sv.run_load(L, C, kinds, Ns, axes, threads=64, rate_qps=0, duration_s=0.2)  # warm-up
+    res = {}
+    th = threading.Thread(target=lambda: res.update(sv.run_load(L, C, kinds, Ns, axes, threads=64, rate_qps=rate,
+                                                                duration_s=1.5, max_outstanding=64)))
+    print(f"load {rate:.0e}/s with {len(probes)} probes", flush=True)
+    th.start()
+    time.sleep(0.3)
+    qids = np.array([10 ** 9 + r[5] for r in probes], np.uint64)
+    t = sv.submit_many([r[0] for r in probes], [r[1] for r in probes], [r[2] for r in probes],
+                       np.stack([r[3] for r in probes]), np.stack([r[4] for r in probes]), qids)
+    mix = sv.wait_many(t)
+    th.join()
+    alone = [sv.wait(sv.submit(r[0], r[1], r[2], r[3], r[4], int(q)), r[0]) for r, q in zip(probes, qids)]
+    sv.close()
+    print(f"offered {rate:.0e}/s: achieved {res['qps']:.4g}/s, p50 {res['p50_ms']:.3f} ms, p99 {res['p99_ms']:.3f} ms, "
+          f"mean batch {res['mean_batch']:.1f}", flush=True)
+    assert abs(res["qps"] / rate - 1) < 0.1
+    assert res["p99_ms"] >= res["p50_ms"] > 0
+    for i, (r, a) in enumerate(zip(probes, alone)):
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(mix[i, :len(a)], a, err_msg=f"probe {i} {r[:3]}")
