@@ -174,14 +174,71 @@ def l1_vs_fdm(solver, L, C, n, torch):
     _lib.check(_lib.lib().drw_solver_eval_kernels(solver.handle, n, dL.data_ptr(), dC.data_ptr(), None,
                                                    probs.data_ptr(), wts.data_ptr(), kern.data_ptr(), None),
                "eval_kernels")
-    pk = D.PackedCubes([23] * n)
-    D.voxelize(L[:n], C[:n], fixed_N=23, normalise=False, out=pk.eps.view(n, 23, 23, 23))
-    D.fdm_solve(pk, 0, tol=1e-10, want_stats=False)
-    torch.cuda.synchronize()
-    G = pk.G.view(n, 6, 529).cpu().numpy()
+    G = fp64_kernels(L, C, n, np.zeros(n, np.int32), torch)
     Gn = probs.cpu().numpy()[:, :, None] * kern.cpu().numpy()
     l1 = np.abs(Gn - G).sum(axis=(1, 2)) / np.abs(G).sum(axis=(1, 2))
     return float(l1.mean()), float(np.percentile(l1, 99))
+
+
+def fp64_kernels(L, C, n, codes, torch):
+    """FDM kernels G [n, 6, 529] at tol 1e-10 (the FP64 accuracy reference),
+    checked to have converged."""
+    from deeprwcap_amd import device as D
+    pk = D.PackedCubes([23] * n)
+    D.voxelize(L[:n], C[:n], fixed_N=23, normalise=False, out=pk.eps.view(n, 23, 23, 23))
+    it, rr = D.fdm_solve(pk, codes, tol=1e-10)
+    torch.cuda.synchronize()
+    it, rr = it.cpu().numpy(), rr.cpu().numpy()
+    if not ((it > 0).all() and (rr <= 1e-10 * (1 + 1e-6)).all()):
+        raise RuntimeError(f"FP64 reference did not converge: min iters {it.min()}, max relres {rr.max():.2e}")
+    return pk.G.view(n, 6, 529).cpu().numpy()
+
+
+def l1_grad_vs_fdm(models_txt, L, C, n, torch):
+    """Gradient query accuracy (dnn_solver_gradient.cpp:111-164 with the
+    targets of dataset.py:114-120): the full signed kernel W p_f k_f of the
+    networks on the six faces, remapped to the query's frame, against the
+    FP64 FDM kernel of the axis' right-hand side (WVTX / WVTY / WVTZ), as
+    sum |G_nn - G_fdm| / sum |G_fdm|; and W against sum |G_fdm|.  Axes cycle
+    0, 1, 2 over the evaluation queries."""
+    from deeprwcap_amd import _lib
+    from deeprwcap_amd.dnnsolver import DNNSolverGrad
+    g = DNNSolverGrad(0)
+    g.loadModels(models_txt=models_txt)
+    axis = (np.arange(n) % 3).astype(np.int32)
+    dL, dC = torch.from_numpy(L[:n]).cuda(), torch.from_numpy(C[:n]).cuda()
+    dA = torch.from_numpy(axis).cuda()
+    probs = torch.empty((n, 6), device="cuda")
+    wts = torch.empty(n, device="cuda")
+    kern = torch.empty((n, 6, 529), device="cuda")
+    _lib.check(_lib.lib().drw_solver_eval_kernels(g.handle, n, dL.data_ptr(), dC.data_ptr(), dA.data_ptr(),
+                                                   probs.data_ptr(), wts.data_ptr(), kern.data_ptr(), None),
+               "eval_kernels")
+    G = fp64_kernels(L, C, n, np.array([{0: 2, 1: 3, 2: 1}[int(a)] for a in axis], np.int32), torch)
+    p, w, k = probs.cpu().numpy(), wts.cpu().numpy(), kern.cpu().numpy()
+    remap = {0: {0: 4, 1: 5, 4: 1, 5: 0}, 1: {0: 2, 1: 3, 2: 1, 3: 0}, 2: {}}  # cuda_kernels.cu:146-153
+    Gn = np.zeros_like(G)
+    for b in range(n):
+        for f in range(6):
+            Gn[b, remap[int(axis[b])].get(f, f)] = w[b] * p[b, f] * k[b, f]
+    tot = np.abs(G).sum(axis=(1, 2))
+    l1 = np.abs(Gn - G).sum(axis=(1, 2)) / tot
+    werr = np.abs(w - tot) / tot
+    return {"l1_grad_vs_fp64_mean": float(l1.mean()), "l1_grad_vs_fp64_p99": float(np.percentile(l1, 99)),
+            "w_rel_err_mean": float(werr.mean()), "w_rel_err_p99": float(np.percentile(werr, 99)),
+            "queries": n, "note": "signed gradient kernel, relative L1 (sum |G_nn - G_fp64| / sum |G_fp64|)"}
+
+
+def host_pointer_rate(solver, L, C, reps=5):
+    """The reference contract's sample() on host vectors: H2D of the
+    descriptions, the query, D2H of the hops and a stream synchronisation
+    (dnn_solver.cpp:264-272, dnn_solver_poisson.cpp:44-62)."""
+    Lf, Cf = np.ascontiguousarray(L.reshape(-1)), np.ascontiguousarray(C.reshape(-1))
+    solver.sample(Lf, Cf)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        solver.sample(Lf, Cf)
+    return len(L) * reps / (time.perf_counter() - t0)
 
 
 def cpu_baseline(L, C, models_txt, seconds=15.0):
@@ -279,17 +336,18 @@ def cap3d_check(models_txt, path, walks):
     return out
 
 
-def serving_runs(models_txt, rates, duration_s=2.0, threads=64):
-    """Serving loop: 64 host threads issue mixed Poisson/gradient N = 23
-    queries with Poisson arrivals; returns per-rate latency and throughput."""
+def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False):
+    """Serving loop: 64 host threads issue mixed Poisson/gradient queries with
+    Poisson arrivals (N = 23, or N uniform over {16, 21, 23} with mixed_n);
+    returns per-rate latency and throughput."""
     from deeprwcap_amd import structures as S
     from deeprwcap_amd.serving import Server
     pool = 4096
-    L, C, _ = S.layered_boxes(pool, 23, seed=99)
     rng = np.random.default_rng(7)
+    Ns = rng.choice(np.array([16, 21, 23], np.int32), pool) if mixed_n else np.full(pool, 23, np.int32)
+    L, C, _ = S.layered_boxes(pool, Ns, seed=99)
     kinds = rng.integers(0, 2, pool).astype(np.int32)
     axes = rng.integers(0, 3, pool).astype(np.int32)
-    Ns = np.full(pool, 23, np.int32)
     sv = Server(0, models_txt, max_batch=1024, max_inflight=4, hbm_budget_mb=512)
     sv.run_load(L, C, kinds, Ns, axes, threads=threads, rate_qps=0, duration_s=0.3)  # warm-up
     out = []
@@ -402,6 +460,11 @@ def main():
     Le, Ce, _ = S.layered_boxes(EVAL_QUERIES, 23, seed=EVAL_SEED)
     l1_mean, l1_p99 = l1_vs_fdm(nn, Le, Ce, EVAL_QUERIES, torch)
     log(f"L1 vs FP64 FDM: mean {l1_mean:.4f} p99 {l1_p99:.4f}")
+    grad_acc = l1_grad_vs_fdm(models_txt, Le, Ce, EVAL_QUERIES, torch)
+    log(f"gradient query vs FP64 FDM: signed-kernel L1 mean {grad_acc['l1_grad_vs_fp64_mean']:.4f} "
+        f"p99 {grad_acc['l1_grad_vs_fp64_p99']:.4f}, W rel err mean {grad_acc['w_rel_err_mean']:.4f}")
+    host_rate = host_pointer_rate(nn, L, C)
+    log(f"host-pointer sample(): {host_rate:,.0f} queries/s (PCIe copies and a synchronisation included)")
 
     # small-batch latency (one synchronous batch of 64 queries)
     lat = []
@@ -436,6 +499,13 @@ def main():
         "fdm_batch": fdm_c2,
         "l1_vs_fp64_mean": l1_mean,
         "l1_vs_fp64_p99": l1_p99,
+        "gate_l1_max": 1e-2,
+        "gate_pass": l1_mean <= 1e-2,
+        "gradient_accuracy": grad_acc,
+        "host_pointer_queries_per_s": host_rate,
+        "precision": "network MFMA operands fp16 (weights and the input layer split into fp16 hi + lo parts), "
+                     "fp32 accumulate and activations where fp16 would exceed the 1e-3 L1 budget; fp16 carries "
+                     "3 more mantissa bits than the bf16 the metric names",
         "latency_ms_batch64": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "stage_ms_per_launch": {"selector_backbone": ms[0] / max(cnt[0], 1), "selector_head": ms[1] / max(cnt[1], 1),
                                 "face_predictor": face_ms},
@@ -448,8 +518,14 @@ def main():
     if not args.no_mixed:
         res["mixed_walkers"] = mixed_walkers(models_txt)
     if not args.no_serving:
-        res["serving"] = serving_runs(models_txt, [1e5, 1e6, 1e7, 0])
-        res["p99_latency_ms"] = res["serving"][1]["p99_ms"]
+        rates = [1e5, 1e6, 1e7, 0]
+        res["serving"] = serving_runs(models_txt, rates)
+        res["p99_latency_ms"] = {("back_to_back" if r["offered_qps"] is None else f"offered_{r['offered_qps']:.0e}"):
+                                 r["p99_ms"] for r in res["serving"]}
+        sustained = [r["offered_qps"] for r in res["serving"] if r["offered_qps"] and r["p99_ms"] < 1.0
+                     and r["achieved_qps"] >= 0.95 * r["offered_qps"]]
+        res["max_offered_qps_with_p99_under_1ms"] = max(sustained) if sustained else None
+        res["serving_mixed_n"] = serving_runs(models_txt, [1e5, 1e6], mixed_n=True)
     if not args.no_cpu:
         log("timing the CPU path ...")
         res["cpu_baseline"] = cpu_baseline(L, C, models_txt)
