@@ -312,27 +312,84 @@ def capacitance_check(models_txt, walks):
     return out
 
 
-def cap3d_check(models_txt, path, walks):
-    """Total capacitance of the master net of a cap3d structure by the walker,
-    with network queries and with FDM-only queries (N = 23), against the
-    reference's field-solver result (ref .dspf of the same name)."""
+def cap3d_reference_file(path):
+    """The field-solver result next to a cap3d structure (.dspf, or .spice
+    for case 10), or None."""
+    stem = path[:-3] if path.endswith(".gz") else path
+    stem = os.path.splitext(stem)[0]
+    for ext in (".dspf", ".spice"):
+        if os.path.exists(stem + ext):
+            return stem + ext
+    return None
+
+
+def cap3d_paths(spec):
+    """--cap3d argument: a structure path, or "all" for the reference's ten
+    test structures kept under tests/golden/cap3d."""
+    if spec != "all":
+        return [spec]
+    d = os.path.join(ROOT, "tests", "golden", "cap3d")
+    out = []
+    for c in range(1, 11):
+        p = os.path.join(d, f"case{c}.cap3d")
+        out.append(p if os.path.exists(p) else p + ".gz")
+    return out
+
+
+def cap3d_check(models_txt, path, walks, fdm=True, threads=32):
+    """Capacitance rows of a cap3d structure's master nets by the walker, from
+    network queries and (fdm) from FDM-only queries at N = 23, scored against
+    the field-solver matrix next to it by the reference's metric
+    (executable/run_script.py:179-197): matrix error = sum |C - C_ref| / sum
+    |C_ref| over the reference rows' entries, self error = |C_mm / C_ref,mm -
+    1| per master.  The masters are the reference's rows (every net the
+    field solver extracted), else the structure's task."""
     from deeprwcap_amd import cap3d
     from deeprwcap_amd.serving import Server
     s = cap3d.read(path)
-    dspf = os.path.splitext(path)[0] + ".dspf"
-    net, ref = cap3d.read_reference_total(open(dspf).read()) if os.path.exists(dspf) else (s["masters"][0], None)
-    out = {"structure": os.path.basename(path), "net": net, "reference_F": ref, "walks": walks}
-    for name, mt in (("network", models_txt), ("fdm", None)):
+    rf = cap3d_reference_file(path)
+    ref = cap3d.read_reference(rf) if rf else None
+    masters = list(ref) if ref else s["masters"]
+    order, _ = cap3d.nets(s)
+    out = {"structure": os.path.basename(path), "masters": masters, "walks_per_master": walks,
+           "conductor_sections": len(s["names"]), "nets": len(order), "boxes": int(len(s["boxes"])),
+           "dielectric_boxes": int(len(s["media_eps"])), "layers": int(len(s["layer_z"]))}
+    log(f"{out['structure']}: {len(masters)} master(s) {masters}, {out['nets']} nets, {out['boxes']} boxes, "
+        f"{out['dielectric_boxes']} dielectric boxes, {out['layers']} layers")
+    backends = (("network", models_txt), ("fdm", None)) if fdm else (("network", models_txt),)
+    for name, mt in backends:
         sv = Server(0, mt, max_batch=4096, max_inflight=4, hbm_budget_mb=8192, fdm_tol=1e-6)
-        t0 = time.perf_counter()
-        c, e, row, st = cap3d.master_capacitance(sv, s, net, walks=walks)
-        dt = time.perf_counter() - t0
+        mat, rows, t_all = {}, {}, time.perf_counter()
+        for m in masters:
+            t0 = time.perf_counter()
+            row, row_err, st = cap3d.master_row(sv, s, m, walks=walks, threads=threads, concurrent=256)
+            dt = time.perf_counter() - t0
+            mat[m] = row
+            rows[m] = {"C_F": row[m], "stderr_F": row_err[m], "seconds": dt, "hops_per_walk": st["hops_per_walk"],
+                       "gauss_area_um2": st["gauss_area"], "gauss_margin_um": st["gauss_margin"],
+                       "window_walks": st["window_walks"], "stalled_walks": st["stalled_walks"],
+                       "self_direct_F": st["self_direct"] * cap3d.EPS0_F_PER_UM,
+                       "self_direct_stderr_F": st["self_direct_stderr"] * cap3d.EPS0_F_PER_UM,
+                       "mean_weight_rel": st["mean_weight"] * cap3d.EPS0_F_PER_UM / row[m],
+                       "mean_weight_rel_stderr": st["mean_weight_stderr"] * cap3d.EPS0_F_PER_UM / row[m]}
+            if ref and m in ref:
+                rows[m]["reference_F"] = ref[m][m]
+                rows[m]["self_error"] = abs(row[m] / ref[m][m] - 1)
+            log(f"  {name} {m}: C {row[m]:.5g} F +- {100 * row_err[m] / row[m]:.2f}%"
+                + (f", reference {ref[m][m]:.5g} F, self error {100 * rows[m]['self_error']:.2f}%" if ref else "")
+                + f" ({dt:.1f} s, {st['hops_per_walk']:.1f} hops/walk, {st['window_walks']:.0f} ended at the window, "
+                  f"{st['stalled_walks']:.0f} stalled; E[w] {100 * rows[m]['mean_weight_rel']:+.1f}% "
+                  f"+- {100 * rows[m]['mean_weight_rel_stderr']:.1f}% of C)")
         sv.close()
-        out[name] = {"C_F": c, "stderr_F": e, "seconds": dt, "hops_per_walk": st["hops_per_walk"],
-                     "vs_reference": None if ref is None else c / ref - 1.0}
-        log(f"{name}: C {c:.5g} F +- {e:.2g} ({dt:.1f} s, {st['hops_per_walk']:.1f} hops/walk)"
-            + ("" if ref is None else f", reference {ref:.5g} F ({100 * (c / ref - 1):+.2f}%)"))
-    out["network_vs_fdm"] = out["network"]["C_F"] / out["fdm"]["C_F"] - 1.0
+        res = {"rows": rows, "seconds": time.perf_counter() - t_all, "matrix": mat}
+        if ref:
+            res["matrix_error"], res["self_error"] = cap3d.matrix_error(mat, ref)
+            log(f"  {name}: matrix error {100 * res['matrix_error']:.2f}%, self errors "
+                + ", ".join(f"{k} {100 * v:.2f}%" for k, v in res["self_error"].items()))
+        out[name] = res
+    if fdm:
+        out["network_vs_fdm_self"] = {m: out["network"]["rows"][m]["C_F"] / out["fdm"]["rows"][m]["C_F"] - 1.0
+                                      for m in masters}
     return out
 
 
@@ -376,10 +433,12 @@ def main():
     ap.add_argument("--capacitance", type=int, default=0, metavar="WALKS",
                     help="only run the end-to-end check: capacitance row of the 2x8-wire structure from network "
                          "queries vs FDM-only queries (N = 23), WALKS walks each")
-    ap.add_argument("--cap3d", metavar="PATH", help="only run the end-to-end check on a cap3d structure: total "
-                    "capacitance of its master net from network queries and from FDM-only queries, beside the "
-                    "field-solver value in the .dspf next to it")
-    ap.add_argument("--walks", type=int, default=20000)
+    ap.add_argument("--cap3d", metavar="PATH", help="only run the end-to-end check on a cap3d structure (or "
+                    "'all': the ten reference structures under tests/golden/cap3d): capacitance rows of its master "
+                    "nets from network queries and from FDM-only queries, scored against the field-solver matrix "
+                    "next to it (matrix and self-capacitance errors); one JSON line per structure")
+    ap.add_argument("--walks", type=int, default=20000, help="walks per master net (--cap3d, --capacitance)")
+    ap.add_argument("--no-fdm-walks", action="store_true", help="--cap3d: network queries only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -404,7 +463,8 @@ def main():
         return
     if args.cap3d:
         models_txt, wkind = models_txt_or_random(args.models)
-        print(json.dumps(cap3d_check(models_txt, args.cap3d, args.walks)), flush=True)
+        for path in cap3d_paths(args.cap3d):
+            print(json.dumps(cap3d_check(models_txt, path, args.walks, fdm=not args.no_fdm_walks)), flush=True)
         return
     if args.capacitance:
         models_txt, wkind = models_txt_or_random(args.models)

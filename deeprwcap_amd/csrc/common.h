@@ -4,11 +4,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "query_limits.h"
+
 namespace drw {
 
-constexpr int kMaxLayers = 15;    // inference_cpp/src/dnn_solver.h:29
-constexpr int kMaxCuboids = 15;   // inference_cpp/src/dnn_solver.h:28
-constexpr float kPad = -9999.0f;  // inference_cpp/src/cuda_kernels.cu:9
 constexpr int kNetN = 23;         // networks exist at N = 23 only (dnn_solver.h:45)
 constexpr int kMaxN = 23;
 constexpr int kMaxCells = kMaxN * kMaxN * kMaxN;  // 12167

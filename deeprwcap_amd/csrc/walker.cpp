@@ -5,8 +5,15 @@
 // an FRW of the standard form its README and query interface imply
 // (README.md:124-139, dnn_solver_gradient.cpp:149-162 for the gradient
 // query's (x, y, z, W sign, eps_centre) result):
-//  * a walk starts at a uniformly drawn point r of a Gaussian surface (the
-//    master's bounding box grown by a margin) with outward normal n;
+//  * the Gaussian surface is the boundary of the union of the master's boxes,
+//    each grown by its own margin (half its gap to the nearest other
+//    conductor or the window), so it encloses the master and nothing else for
+//    any box layout; a draw picks a face of the grown boxes with probability
+//    proportional to area / margin and a uniform point on it, and is kept
+//    when the point lies on the union's boundary (then the face's outward
+//    normal n is the surface's); a kept draw weighs A = Z margin (Z = sum of
+//    area / margin over the faces), a rejected one counts as a zero-weight
+//    walk;
 //  * the first hop is a gradient query along n's axis on the largest cube
 //    centred at r that touches no conductor; it returns the next point and
 //    W sign(g), so that d phi/dn(r) ~ (s/h) W sign(g) phi(next) (h = half
@@ -14,123 +21,50 @@
 //  * further hops are Poisson queries on the largest conductor-free cube
 //    around the current point; a walk ends within `stop_dist` of a conductor
 //    (phi = 1 for that conductor's column) or of the window (phi = 0);
-//  * C[master][k] = -E[ A eps(r) (s/h) W sign 1(end on k) ], A = area of the
-//    Gaussian surface.
+//  * with the walk weight w = -A eps(r) (s/h) W sign, C[master][k] = E[ w
+//    1(end on k) ] over all draws for k != master, and the master's own
+//    entry comes from charge conservation, C[m][m] = -E[ w 1(end on another
+//    conductor or the window) ]: the gradient kernel sums to zero (the
+//    gradient of a constant), so E[w] = 0 and the two forms agree, but the
+//    walks that return to the master (most of them where the surface hugs the
+//    master across a narrow gap) add nothing to the second, whose variance is
+//    then several times smaller; it also does not multiply a kernel's
+//    residual sum by the returning walks.  The direct form and E[w] are
+//    reported beside it.
 // Dielectrics are planar z-layers; each cube's description lists the layers
 // it crosses in its own normalised coordinates (dnn_solver.h:27-30).
+// Conductor and dielectric boxes are held in uniform grids, so a hop's
+// geometry queries visit the cells near the walk instead of every box.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <string>
+#include <limits>
 #include <mutex>
+#include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/dnnsolver_c.h"
 #include "common.h"
 #include "runtime.h"
+#include "walker_geometry.h"
 
 namespace {
 
-struct Geo {
-    double win[6];
-    std::vector<double> lz, leps;  // layer bottoms (ascending) and permittivities
-    std::vector<double> box;       // [n][6] conductor boxes
-    std::vector<int> id;           // conductor id per box
-    std::vector<double> mbox;      // [m][6] dielectric boxes (override the layers, first listed wins)
-    std::vector<double> meps;      // [m]
-    int ncond = 0;
-};
-
-// L-infinity distance from p to an axis-aligned box (0 inside).
-double box_dist(const double* b, const double* p)
-{
-    double d = 0;
-    for (int a = 0; a < 3; a++) {
-        const double lo = b[a] - p[a], hi = p[a] - b[a + 3];
-        d = std::max(d, std::max(lo, hi));
-    }
-    return std::max(d, 0.0);
-}
-
-double eps_at(const Geo& g, double z)
-{
-    int i = 0;
-    for (size_t k = 0; k < g.lz.size(); k++)
-        if (z >= g.lz[k]) i = (int)k;
-    return g.leps[i];
-}
-
-// Largest half side <= h for which the cube centred at c crosses at most
-// kMaxLayers - 1 layer interfaces and overlaps at most kMaxCuboids
-// dielectric boxes (the limits of a query's structure description,
-// dnn_solver.h:28-29): h drops to the distance of the first element past the
-// limit, which then only touches the cube.
-double fit_cube(const Geo& g, const double* c, double h, std::vector<double>& tmp)
-{
-    tmp.clear();
-    for (size_t k = 1; k < g.lz.size(); k++) {
-        const double d = std::fabs(g.lz[k] - c[2]);
-        if (d < h) tmp.push_back(d);
-    }
-    if ((int)tmp.size() > drw::kMaxLayers - 1) {
-        std::nth_element(tmp.begin(), tmp.begin() + (drw::kMaxLayers - 1), tmp.end());
-        h = tmp[drw::kMaxLayers - 1];
-    }
-    tmp.clear();
-    for (size_t i = 0; i < g.meps.size(); i++) {
-        const double d = box_dist(&g.mbox[6 * i], c);
-        if (d < h) tmp.push_back(d);
-    }
-    if ((int)tmp.size() > drw::kMaxCuboids) {
-        std::nth_element(tmp.begin(), tmp.begin() + drw::kMaxCuboids, tmp.end());
-        h = tmp[drw::kMaxCuboids];
-    }
-    return h;
-}
-
-// Structure description of the cube centred at c with half side h, in the
-// cube's normalised coordinates (dnn_solver.h:27-30): the layers it crosses
-// and the dielectric boxes it overlaps (first listed wins, as in the
-// voxelizer).
-void describe(const Geo& g, const double* c, double h, float* layer, float* cub)
-{
-    for (int i = 0; i < 30; i++) layer[i] = drw::kPad;
-    for (int i = 0; i < 105; i++) cub[i] = drw::kPad;
-    int n = 0;
-    layer[0] = -1.5f;
-    layer[1] = (float)eps_at(g, c[2] - h);
-    n = 1;
-    for (size_t k = 0; k < g.lz.size() && n < drw::kMaxLayers; k++) {
-        const double zn = (g.lz[k] - c[2]) / h;
-        if (zn > -1.0 && zn < 1.0) {
-            layer[2 * n] = (float)zn;
-            layer[2 * n + 1] = (float)g.leps[k];
-            n++;
-        }
-    }
-    int m = 0;
-    for (size_t i = 0; i < g.meps.size() && m < drw::kMaxCuboids; i++) {
-        const double* b = &g.mbox[6 * i];
-        if (!(box_dist(b, c) < h)) continue;
-        for (int a = 0; a < 3; a++) {
-            cub[7 * m + a] = (float)std::max((b[a] - c[a]) / h, -1.5);
-            cub[7 * m + 3 + a] = (float)std::min((b[a + 3] - c[a]) / h, 1.5);
-        }
-        cub[7 * m + 6] = (float)g.meps[i];
-        m++;
-    }
-}
+using namespace drw::walkgeo;
 
 struct Walk {
     double p[3];
     double w = 0;  // -A eps (s/h) W sign
     int hops = 0;
     bool first = true;
+    bool stalled = false;  // no cube of at least the floor size fits
     int axis = 2;
-    double sgn = 1, area_eps = 0, h = 0;
+    double sgn = 1, h = 0;
+    double area = 0;  // the start draw's weight (Z margin of its face)
 };
 
 }  // namespace
@@ -140,6 +74,8 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
 {
     drw::clear_error();
     if (!sv || !geo || !wc || !cap || !stderr_out) return drw::set_error(DRW_EINVAL, "walk: null argument");
+    if (geo->n_layers < 1 || geo->n_boxes < 1 || !geo->layer_z || !geo->layer_eps || !geo->cond_box || !geo->cond_id)
+        return drw::set_error(DRW_EINVAL, "walk: bad geometry");
     Geo g;
     std::memcpy(g.win, geo->window, sizeof g.win);
     g.lz.assign(geo->layer_z, geo->layer_z + geo->n_layers);
@@ -151,52 +87,34 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
         g.mbox.assign(geo->media_box, geo->media_box + 6 * geo->n_media);
         g.meps.assign(geo->media_eps, geo->media_eps + geo->n_media);
     }
-    for (int v : g.id) g.ncond = std::max(g.ncond, v + 1);
-    if (geo->n_layers < 1 || geo->n_boxes < 1 || master < 0 || master >= g.ncond)
-        return drw::set_error(DRW_EINVAL, "walk: bad geometry or master");
-    if (wc->walks < 1 || wc->threads < 1 || wc->concurrent < 1 || !(wc->stop_dist > 0) || !(wc->max_cube > 0))
-        return drw::set_error(DRW_EINVAL, "walk: bad configuration");
-    // master bounding box and Gaussian surface margin
-    double mb[6] = {1e300, 1e300, 1e300, -1e300, -1e300, -1e300};
-    for (size_t i = 0; i < g.id.size(); i++)
-        if (g.id[i] == master)
-            for (int a = 0; a < 3; a++) {
-                mb[a] = std::min(mb[a], g.box[6 * i + a]);
-                mb[a + 3] = std::max(mb[a + 3], g.box[6 * i + a + 3]);
-            }
-    double gap = 1e300;
-    for (size_t i = 0; i < g.id.size(); i++) {
-        if (g.id[i] == master) continue;
-        double d = 0;
-        for (int a = 0; a < 3; a++) d = std::max(d, std::max(g.box[6 * i + a] - mb[a + 3], mb[a] - g.box[6 * i + a + 3]));
-        gap = std::min(gap, d);
+    for (int v : g.id) {
+        if (v < 0) return drw::set_error(DRW_EINVAL, "walk: negative conductor id");
+        g.ncond = std::max(g.ncond, v + 1);
     }
-    const double margin = wc->gauss_frac * (gap < 1e299 ? gap : wc->max_cube);
-    double G[6];
-    for (int a = 0; a < 3; a++) { G[a] = mb[a] - margin; G[a + 3] = mb[a + 3] + margin; }
-    const double ext[3] = {G[3] - G[0], G[4] - G[1], G[5] - G[2]};
-    const double face_area[3] = {ext[1] * ext[2], ext[0] * ext[2], ext[0] * ext[1]};
-    const double area = 2 * (face_area[0] + face_area[1] + face_area[2]);
+    if (master < 0 || master >= g.ncond) return drw::set_error(DRW_EINVAL, "walk: bad master %d", master);
+    if (wc->walks < 1 || wc->threads < 1 || wc->concurrent < 1 || !(wc->stop_dist > 0) || !(wc->max_cube > 0) ||
+        !(wc->gauss_frac > 0 && wc->gauss_frac < 1))
+        return drw::set_error(DRW_EINVAL, "walk: bad configuration");
+    g.cgrid.build(g.box.data(), g.id.size());
+    g.mgrid.build(g.mbox.data(), g.meps.size());
+
+    Gauss gs;
+    const std::string gerr = build_gauss(g, master, wc->gauss_frac, wc->max_cube, gs);
+    if (!gerr.empty()) return drw::set_error(DRW_EINVAL, "walk: %s", gerr.c_str());
 
     std::vector<double> sum(g.ncond, 0.0), sum2(g.ncond, 0.0);
-    std::atomic<long long> hops_total{0}, walks_done{0};
+    std::atomic<long long> hops_total{0}, walks_done{0}, draws_total{0}, window_total{0}, stalled_total{0};
+    double area_total = 0, cons = 0, cons2 = 0, wsum = 0, wsum2 = 0;  // under mu
+    // smallest cube worth a query: far below any feature, far above rounding
+    const double h_floor = 1e-12 * std::max(g.win[3] - g.win[0], std::max(g.win[4] - g.win[1], g.win[5] - g.win[2]));
+    constexpr int kMaxHops = 100000;
     std::atomic<int> failed{0};
     std::mutex mu;
-    std::string first_bad;  // description of the first non-finite query result
+    std::string first_bad;  // description of the first failure
     const int N = wc->N;
 
-    auto dist_all = [&](const double* p, int* nearest) {
-        double d = 1e300;
-        int k = -1;
-        for (size_t i = 0; i < g.id.size(); i++) {
-            const double di = box_dist(&g.box[6 * i], p);
-            if (di < d) { d = di; k = g.id[i]; }
-        }
-        *nearest = k;
-        return d;
-    };
     auto win_dist = [&](const double* p) {
-        double d = 1e300;
+        double d = kInf;
         for (int a = 0; a < 3; a++) d = std::min(d, std::min(p[a] - g.win[a], g.win[a + 3] - p[a]));
         return d;
     };
@@ -208,72 +126,98 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
             return (double)((rng * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
         };
         const long long quota = wc->walks / wc->threads + (tid < wc->walks % wc->threads ? 1 : 0);
-        long long started = 0;
+        long long started = 0, draws = 0;
+        double area_sum = 0;  // weights of the accepted draws
         std::vector<double> lsum(g.ncond, 0.0), lsum2(g.ncond, 0.0);
-        long long lhops = 0, ldone = 0;
+        long long lhops = 0, ldone = 0, lwindow = 0, lstalled = 0;
+        double lcons = 0, lcons2 = 0, lw = 0, lw2 = 0;
+        Scratch sc;
+        sc.stamp.assign(g.meps.size(), 0u);
+        // draws points until one lies on the Gaussian surface; every draw is
+        // one sample of the estimator (the rejected ones weigh zero)
         auto start = [&](Walk& w) {
-            // area-weighted face, uniform point
-            double u = U() * area * 0.5;
-            int ax = 0;
-            while (ax < 2 && u > face_area[ax]) { u -= face_area[ax]; ax++; }
-            w.sgn = U() < 0.5 ? -1.0 : 1.0;
-            for (int a = 0; a < 3; a++) w.p[a] = G[a] + U() * ext[a];
-            w.p[ax] = w.sgn < 0 ? G[ax] : G[ax + 3];
-            w.axis = ax;
-            w.first = true;
-            w.hops = 0;
-            w.w = 0;
-            started++;
+            for (long long tries = 0;; tries++) {
+                if (tries > 10000000) return false;
+                draws++;
+                const double u = U(), v[3] = {U(), U(), U()};
+                int a = 0, s = 0;
+                if (!gauss_draw(gs, u, v, w.p, &a, &s, &w.area)) continue;
+                area_sum += w.area;
+                w.sgn = s ? 1.0 : -1.0;
+                w.axis = a;
+                w.first = true;
+                w.stalled = false;
+                w.hops = 0;
+                w.w = 0;
+                started++;
+                return true;
+            }
+        };
+        auto fail = [&](const char* msg) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first_bad.empty()) first_bad = msg;
+            failed++;
         };
         std::vector<Walk> ws(wc->concurrent);
         std::vector<char> active(wc->concurrent, 0);
-        for (int i = 0; i < wc->concurrent && started < quota; i++) { start(ws[i]); active[i] = 1; }
+        for (int i = 0; i < wc->concurrent && started < quota; i++) {
+            if (!start(ws[i])) { fail("walk: no point of the Gaussian surface found"); return; }
+            active[i] = 1;
+        }
         // one submission and one wait per round for all walks of this thread
         const int W = wc->concurrent;
         std::vector<int32_t> kk(W), nn(W), aa(W);
         std::vector<float> LL((size_t)W * 30), CC((size_t)W * 105), outs((size_t)W * 5);
         std::vector<uint64_t> tk(W);
         std::vector<int> idx;
-        std::vector<double> tmp;
         for (;;) {
             idx.clear();
             for (int i = 0; i < W; i++) {
                 if (!active[i]) continue;
                 Walk& w = ws[i];
                 int near = -1;
-                double dc = dist_all(w.p, &near);
                 double dw = win_dist(w.p);
-                if (!w.first && (dc < wc->stop_dist || dw < wc->stop_dist || w.hops > 100000)) {
-                    if (dc < wc->stop_dist && near >= 0) {
-                        lsum[near] += w.w;
-                        lsum2[near] += w.w * w.w;
+                double enough = std::max(std::min(dw, wc->max_cube), wc->stop_dist);
+                double dc = nearest_conductor(g, w.p, enough, &near);
+                if (w.stalled || (!w.first && (dc < wc->stop_dist || dw < wc->stop_dist || w.hops > kMaxHops))) {
+                    if (w.stalled || w.hops > kMaxHops) lstalled++;
+                    else {
+                        const bool on = dc < wc->stop_dist && near >= 0;
+                        if (on) {
+                            lsum[near] += w.w;
+                            lsum2[near] += w.w * w.w;
+                        } else lwindow++;
+                        if (!on || near != master) {  // charge conservation: -w off the master
+                            lcons -= w.w;
+                            lcons2 += w.w * w.w;
+                        }
+                        lw += w.w;
+                        lw2 += w.w * w.w;
                     }
                     lhops += w.hops;
                     ldone++;
                     if (started >= quota) { active[i] = 0; continue; }
                     // the slot starts a new walk: its first cube is sized at the new point
-                    start(w);
-                    dc = dist_all(w.p, &near);
+                    if (!start(w)) {
+                        fail("walk: no point of the Gaussian surface found");
+                        active[i] = 0;
+                        continue;
+                    }
                     dw = win_dist(w.p);
+                    enough = std::max(std::min(dw, wc->max_cube), wc->stop_dist);
+                    dc = nearest_conductor(g, w.p, enough, &near);
                 }
                 double h = std::min(std::min(dc, dw), wc->max_cube);
-                if (w.first) h = std::min(h, margin);
-                h = fit_cube(g, w.p, h, tmp);
-                if (!(h > 0)) {  // a start point on a conductor or the window: geometry error
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (first_bad.empty()) {
-                        char msg[160];
-                        std::snprintf(msg, sizeof msg, "walk at (%.6g, %.6g, %.6g) has no room for a cube", w.p[0],
-                                      w.p[1], w.p[2]);
-                        first_bad = msg;
-                    }
-                    failed++;
-                    active[i] = 0;
+                h = fit_cube(g, w.p, h, sc);
+                if (!(h >= h_floor)) {
+                    // more boxes than a description holds meet at this point
+                    // (or a degenerate cube): the walk ends here, counted
+                    w.stalled = true;
                     continue;
                 }
                 w.h = h;
                 const int j = (int)idx.size();
-                describe(g, w.p, h, &LL[(size_t)j * 30], &CC[(size_t)j * 105]);
+                describe(g, w.p, h, sc, &LL[(size_t)j * 30], &CC[(size_t)j * 105]);
                 kk[j] = w.first ? DRW_GRADIENT : DRW_POISSON;
                 nn[j] = N;
                 aa[j] = w.axis;
@@ -296,26 +240,21 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
                 bool finite = true;
                 for (int k = 0; k < nout; k++) finite = finite && std::isfinite(out[k]);
                 if (!finite) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (first_bad.empty()) {
-                        char msg[256];
-                        std::snprintf(msg, sizeof msg,
-                                      "%s query at (%.6g, %.6g, %.6g) half side %.6g axis %d returned (%g, %g, %g%s",
-                                      w.first ? "gradient" : "Poisson", w.p[0], w.p[1], w.p[2], w.h, w.axis,
-                                      (double)out[0], (double)out[1], (double)out[2], w.first ? "" : ")");
-                        first_bad = msg;
-                        if (w.first) {
-                            std::snprintf(msg, sizeof msg, ", %g, %g)", (double)out[3], (double)out[4]);
-                            first_bad += msg;
-                        }
-                    }
-                    failed++;
+                    char msg[256];
+                    int len = std::snprintf(msg, sizeof msg,
+                                            "%s query at (%.6g, %.6g, %.6g) half side %.6g axis %d returned (%g, %g, %g",
+                                            w.first ? "gradient" : "Poisson", w.p[0], w.p[1], w.p[2], w.h, w.axis,
+                                            (double)out[0], (double)out[1], (double)out[2]);
+                    if (len > 0 && len < (int)sizeof msg)
+                        std::snprintf(msg + len, sizeof msg - len, w.first ? ", %g, %g)" : ")", (double)out[3],
+                                      (double)out[4]);
+                    fail(msg);
                     active[idx[j]] = 0;
                     continue;
                 }
                 if (w.first) {
-                    // -A eps (s/h) W sign, applied when the walk ends
-                    w.w = -area * (double)out[4] * w.sgn / w.h * (double)out[3];
+                    // -A eps (s/h) W sign, applied when the walk ends (A: the draw's weight)
+                    w.w = -w.area * (double)out[4] * w.sgn / w.h * (double)out[3];
                     w.first = false;
                 }
                 for (int a = 0; a < 3; a++) w.p[a] += w.h * (double)out[a];
@@ -326,6 +265,14 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
         for (int k = 0; k < g.ncond; k++) { sum[k] += lsum[k]; sum2[k] += lsum2[k]; }
         hops_total += lhops;
         walks_done += ldone;
+        draws_total += draws;
+        window_total += lwindow;
+        stalled_total += lstalled;
+        area_total += area_sum;
+        cons += lcons;
+        cons2 += lcons2;
+        wsum += lw;
+        wsum2 += lw2;
     };
     std::vector<std::thread> th;
     for (int t = 0; t < wc->threads; t++) th.emplace_back(worker, t);
@@ -333,17 +280,29 @@ extern "C" int drw_walk_capacitance(drw_server* sv, const drw_geometry* geo, int
     if (failed)
         return drw::set_error(DRW_ESTATE, "walk: %d queries failed: %s", (int)failed,
                               first_bad.empty() ? drw::last_error() : first_bad.c_str());
-    const double M = (double)walks_done.load();
-    for (int k = 0; k < g.ncond; k++) {
-        const double m = sum[k] / M, v = std::max(sum2[k] / M - m * m, 0.0);
-        cap[k] = m;
-        stderr_out[k] = std::sqrt(v / M);
-    }
+    const double M = (double)draws_total.load();
+    auto mean_err = [M](double s1, double s2, double* m, double* e) {
+        *m = s1 / M;
+        *e = std::sqrt(std::max(s2 / M - *m * *m, 0.0) / M);
+    };
+    for (int k = 0; k < g.ncond; k++) mean_err(sum[k], sum2[k], &cap[k], &stderr_out[k]);
+    double direct, direct_err, wm, wm_err;
+    direct = cap[master];
+    direct_err = stderr_out[master];
+    mean_err(cons, cons2, &cap[master], &stderr_out[master]);
+    mean_err(wsum, wsum2, &wm, &wm_err);
     if (stats) {
-        stats[0] = M;
-        stats[1] = (double)hops_total.load() / std::max(M, 1.0);
-        stats[2] = area;
-        stats[3] = margin;
+        const double done = (double)walks_done.load();
+        stats[0] = done;
+        stats[1] = (double)hops_total.load() / std::max(done, 1.0);
+        stats[2] = area_total / std::max(M, 1.0);  // the surface's area, estimated by the draws
+        stats[3] = gs.min_margin;
+        stats[4] = (double)window_total.load();
+        stats[5] = (double)stalled_total.load();
+        stats[6] = direct;
+        stats[7] = direct_err;
+        stats[8] = wm;
+        stats[9] = wm_err;
     }
     return 0;
 }

@@ -139,10 +139,12 @@ class Server:
         nc = int(ids.max()) + 1
         cap = np.zeros(nc)
         err = np.zeros(nc)
-        st = np.zeros(4)
+        st = np.zeros(10)
         check(_lib.lib().drw_walk_capacitance(self._h, ctypes.byref(g), master, ctypes.byref(wc), cap.ctypes.data,
                                               err.ctypes.data, st.ctypes.data), "drw_walk_capacitance")
-        return cap, err, {"walks": st[0], "hops_per_walk": st[1], "gauss_area": st[2], "gauss_margin": st[3]}
+        return cap, err, {"walks": st[0], "hops_per_walk": st[1], "gauss_area": st[2], "gauss_margin": st[3],
+                          "window_walks": st[4], "stalled_walks": st[5], "self_direct": st[6],
+                          "self_direct_stderr": st[7], "mean_weight": st[8], "mean_weight_stderr": st[9]}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -261,12 +261,24 @@ int drw_server_run_load(drw_server* sv, const drw_load_config* load, const float
  * Floating-random-walk capacitance extraction (the consumer of the query
  * path; stands in for the closed walker executable/bin/deepRWCap).  Planar
  * z-layers of dielectric, optional dielectric boxes over them, conductors
- * as unions of boxes, Dirichlet 0 on the window.  Walks start on a Gaussian surface around `master`, take one
- * gradient query then Poisson queries through `sv` until they come within
- * stop_dist of a conductor; cap[k] receives C[master][k] (the units of the
- * coordinates times eps; multiply by eps0 for farads), stderr_out[k] its
- * standard error; stats (optional, 4 doubles): walks, mean hops per walk,
- * Gaussian-surface area, Gaussian margin.
+ * as unions of boxes (cond_id: the conductor, i.e. net, of each box),
+ * Dirichlet 0 on the window.  Walks start on a Gaussian surface around
+ * `master` (the boundary of the union of its boxes, each grown by gauss_frac
+ * of its gap to the nearest other conductor or the window; faces drawn in
+ * proportion to area / margin), take one gradient query then Poisson queries
+ * through `sv` until they come within stop_dist of a conductor; cap[k]
+ * receives C[master][k] (the units of the coordinates times eps; multiply by
+ * eps0 for farads), the master's own entry from charge conservation (minus
+ * the flux of the walks that end on another conductor or the window),
+ * stderr_out[k] its standard error; stats (optional, 10 doubles): walks,
+ * mean hops per walk, Gaussian-surface area, smallest Gaussian margin, walks
+ * that ended at the window, walks that ended elsewhere (where more
+ * dielectric boxes meet than a description holds, no cube of 1e-12 of the
+ * window fits; or after 100,000 hops), the master's entry estimated
+ * directly (the flux of the walks that return to it) and its standard
+ * error, the mean walk weight E[w] (zero for kernels that sum to zero: the
+ * difference of the two estimates) and its standard error.  Fails
+ * (DRW_EINVAL) when a master box touches another conductor or the window.
  * ------------------------------------------------------------------------ */
 typedef struct {
     double window[6];          /* xmin ymin zmin xmax ymax zmax */
@@ -286,7 +298,7 @@ typedef struct {
     int concurrent;            /* walks in flight per thread */
     int N;                     /* cube resolution (23: networks when the server has models) */
     double stop_dist;          /* termination distance to a conductor / the window */
-    double gauss_frac;         /* Gaussian margin as a fraction of the smallest gap */
+    double gauss_frac;         /* Gaussian margin of each master box as a fraction of its gap (0..1) */
     double max_cube;           /* largest cube half side */
     uint64_t seed;
 } drw_walk_config;

@@ -50,3 +50,75 @@ def test_rejects_skewed_block():
 <conductor> name A <block> basepoint(0,0,0) v1(1,1,0) v2(0,1,0) hvector(0,0,1) </block> </conductor></cap3d>"""
     with pytest.raises(ValueError):
         cap3d.parse(text)
+
+
+def test_net_names():
+    assert cap3d.net_of("7__signalGND") == "signalGND"
+    assert cap3d.net_of("ln_13") == "ln_13" and cap3d.net_of("a__b") == "a__b" and cap3d.net_of("3__") == "3__"
+    s = cap3d.read(os.path.join(HERE, "case9.cap3d.gz"))
+    order, ids = cap3d.nets(s)
+    # 47 conductor sections, 18 nets (ln_13 and others listed in several sections)
+    assert len(s["names"]) == 47 and len(order) == 18 and s["names"].count("ln_13") > 1
+    assert ids.shape == s["ids"].shape and set(ids) == set(range(18))
+    for k, name in enumerate(s["names"]):
+        assert (ids[s["ids"] == k] == order.index(cap3d.net_of(name))).all()
+
+
+@pytest.mark.parametrize("case", range(1, 11))
+def test_reference_nets_exist(case):
+    """Every net a reference row names is a net of the structure."""
+    p = os.path.join(HERE, f"case{case}.cap3d")
+    s = cap3d.read(p if os.path.exists(p) else p + ".gz")
+    ref = cap3d.read_reference(os.path.join(HERE, f"case{case}.dspf" if case < 10 else "case10.spice"))
+    order, _ = cap3d.nets(s)
+    assert {n for row in ref.values() for n in row} <= set(order)
+    assert set(ref) <= set(order)
+
+
+def test_reference_matrix_dspf():
+    ref = cap3d.read_reference(os.path.join(HERE, "case7.dspf"))
+    assert list(ref) == ["1", "2", "4", "5", "6"]
+    assert ref["1"]["1"] == 9.176692675804292e-16
+    # a coupling in both rows is their mean, in both rows
+    m = (1.849878959329723e-17 + 1.8502064456341546e-17) / 2
+    assert ref["1"]["2"] == ref["2"]["1"] == m
+    assert ref["1"]["3"] == 1.6741187825602342e-17 and "3" not in ref
+    assert set(ref["1"]) == {"1", "2", "3", "4", "5", "6", "GROUND", "signalGND"}
+    # the field solver's totals are the sums of their rows' couplings
+    for k, row in ref.items():
+        assert sum(v for n, v in row.items() if n != k) == pytest.approx(row[k], rel=2e-3)
+    text = "*|NET A 1e-15\nC1 A 0 2e-16\nC2 A B 3e-16\n"
+    assert cap3d.read_reference_matrix(text) == {"A": {"A": 1e-15, "GROUND": 2e-16, "B": 3e-16}}
+
+
+def test_reference_matrix_spice():
+    ref = cap3d.read_reference(os.path.join(HERE, "case10.spice"))
+    assert list(ref) == ["G"] and len(ref["G"]) == 8
+    assert ref["G"]["G"] == pytest.approx(1.241e-15, rel=1e-12)
+    assert ref["G"]["S"] == pytest.approx(0.5735e-15, rel=1e-12)
+    assert ref["G"]["ln_14"] == 2.44e-18
+    assert cap3d.read_reference_spice("** X 2pF * (total)\nCp1 X 0 1.5fF\n") == \
+        {"X": {"X": pytest.approx(2e-12), "GROUND": pytest.approx(1.5e-15)}}
+
+
+def test_matrix_error():
+    ref = cap3d.read_reference(os.path.join(HERE, "case7.dspf"))
+    err, self_err = cap3d.matrix_error(ref, ref)
+    assert err == 0 and self_err == {k: 0 for k in ref}
+    total = sum(abs(v) for row in ref.values() for v in row.values())
+    # every entry 2 % high
+    hi = {m: {n: 1.02 * v for n, v in row.items()} for m, row in ref.items()}
+    err, self_err = cap3d.matrix_error(hi, ref)
+    assert err == pytest.approx(0.02) and all(v == pytest.approx(0.02) for v in self_err.values())
+    # a missing coupling counts as zero
+    drop = {m: dict(row) for m, row in ref.items()}
+    del drop["1"]["3"]
+    assert cap3d.matrix_error(drop, ref)[0] == pytest.approx(ref["1"]["3"] / total)
+    # rows the matrix does not hold are left out of both sums
+    one = {"1": {n: 1.1 * v for n, v in ref["1"].items()}}
+    err, self_err = cap3d.matrix_error(one, ref)
+    assert err == pytest.approx(0.1) and list(self_err) == ["1"]
+    # extra entries of the matrix are not scored
+    more = {m: dict(row, extra=1.0) for m, row in ref.items()}
+    assert cap3d.matrix_error(more, ref)[0] == 0
+    assert np.isnan(cap3d.matrix_error(None, ref)[0])
