@@ -387,6 +387,99 @@ def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda", weighted=False)
     return float(np.mean(errs))
 
 
+# --------------------------------------------------------------------------
+# GGFT dataset files (poisson.bin / gradient.bin, the reference's training
+# input): a float64 header [N, block_width], then per sample the N^3
+# permittivities (x fastest), GGFT_STRUCTURES blocks of 7 structure values and
+# the 6 N^2 kernel values, all float64 (ggft/src/ggft_solver.cpp:35-48,
+# ggft_solver.h:180-191, ggft/src/main.cpp:59-78; read by
+# training_pytorch/src/dataset.py:55-86 with n_structures = 16 and by
+# DNNSolver::readBinaryFile with structureSize = 112).
+
+GGFT_STRUCTURES = 16  # 15 blocks + the background (dielectric_generator.h:55-79)
+
+
+def description_blocks(layers, cuboids, n=GGFT_STRUCTURES):
+    """Structure blocks [B, n, 7] of descriptions in the generator's record
+    form (centre x, y, z, half-extent x, y, z, value; dielectric_generator.h:
+    8-19, 95-106), first matching block wins as there: the cuboids in listing
+    order, then each layer as a slab over the whole cube (x, y half-extent 2,
+    the generator's span; the first layer reaches down to -2), zero blocks
+    after."""
+    L = np.asarray(layers, np.float64).reshape(len(layers), -1, 2)
+    C = np.asarray(cuboids, np.float64).reshape(len(cuboids), -1, 7)
+    out = np.zeros((len(L), n, 7))
+    for b in range(len(L)):
+        rows = []
+        for c in C[b]:
+            if c[0] == S.PAD:
+                break
+            lo, hi = np.maximum(c[:3], -2.0), np.minimum(c[3:6], 2.0)
+            rows.append([*(lo + hi) / 2, *(hi - lo) / 2, c[6]])
+        lay = [l for l in L[b] if l[0] != S.PAD]
+        for k, (z0, e) in enumerate(lay):
+            lo = -2.0 if k == 0 else max(z0, -2.0)  # the first layer holds everything below the next
+            hi = min(lay[k + 1][0], 2.0) if k + 1 < len(lay) else 2.0
+            rows.append([0.0, 0.0, (lo + hi) / 2, 2.0, 2.0, (hi - lo) / 2, e])
+        if len(rows) > n:
+            raise ValueError(f"description {b} needs {len(rows)} structure blocks, a record holds {n}")
+        out[b, :len(rows)] = rows
+    return out
+
+
+def write_ggft_bin(path, cubes, G, blocks=None, N=NET_N, block_width=1, append=False):
+    """Write (or append, without a second header) GGFT dataset records:
+    cubes [n, N, N, N] (x fastest), blocks [n, GGFT_STRUCTURES, 7] (zeros when
+    None), G [n, 6, N*N], all stored as float64."""
+    cubes = np.asarray(cubes).reshape(-1, N ** 3)
+    G = np.asarray(G).reshape(len(cubes), 6 * N * N)
+    if blocks is None:
+        blocks = np.zeros((len(cubes), GGFT_STRUCTURES, 7))
+    blocks = np.asarray(blocks, np.float64).reshape(len(cubes), -1)
+    rec = np.concatenate([cubes.astype(np.float64), blocks, G.astype(np.float64)], axis=1)
+    with open(path, "ab" if append else "wb") as f:
+        if not append:
+            f.write(np.array([N, block_width], np.float64).tobytes())
+        f.write(np.ascontiguousarray(rec).tobytes())
+
+
+def read_ggft_bin(path, N=NET_N, n_structures=GGFT_STRUCTURES):
+    """(cubes [n, N, N, N], blocks [n, n_structures, 7], G [n, 6, N*N]),
+    float64, memory-mapped, from a GGFT dataset file (dataset.py:55-86)."""
+    v = np.memmap(path, dtype=np.float64, mode="r")
+    n, bw = int(v[0]), int(v[1])
+    if n != N or N % bw:
+        raise ValueError(f"{path}: header N={n}, block width {bw}; expected N={N}")
+    bn = N // bw
+    rec = bn ** 3 + 7 * n_structures + 6 * N * N
+    if (v.size - 2) % rec:
+        raise ValueError(f"{path}: {v.size - 2} values are no whole number of {rec}-value records")
+    body = v[2:].reshape(-1, rec)
+    return (body[:, :bn ** 3].reshape(-1, bn, bn, bn), body[:, bn ** 3:bn ** 3 + 7 * n_structures]
+            .reshape(-1, n_structures, 7), body[:, -6 * N * N:].reshape(-1, 6, N * N))
+
+
+def ggft_bin_dataset(out_dir, n, seed, tol=1e-6, max_boxes=2, chunk=4096, dev="cuda", log=print):
+    """poisson.bin (G^GFT) and gradient.bin (G^z, code WVTZ) of n seeded
+    descriptions under out_dir, as the reference's GGFT driver writes them
+    (ggft/run_ggft.sh, main.cpp:38-78: solver tolerance 1e-6), solved by the
+    library's FDM on the GPU; DIEL holds the unnormalised permittivities."""
+    os.makedirs(out_dir, exist_ok=True)
+    paths = {0: os.path.join(out_dir, "poisson.bin"), 1: os.path.join(out_dir, "gradient.bin")}
+    for c0 in range(0, n, chunk):
+        b = min(chunk, n - c0)
+        L, C, _ = S.layered_boxes(b, NET_N, seed=1_000_003 * (seed + 1) + c0, max_boxes=max_boxes)
+        blocks = description_blocks(L, C)
+        pk = D.PackedCubes([NET_N] * b, device=dev)
+        D.voxelize(L, C, fixed_N=NET_N, normalise=False, out=pk.eps.view(b, NET_N, NET_N, NET_N))
+        cubes = pk.eps.view(b, NET_N, NET_N, NET_N).cpu().numpy()
+        for code in (0, 1):
+            D.fdm_solve(pk, code, tol=tol, want_stats=False)
+            write_ggft_bin(paths[code], cubes, pk.G.view(b, 6, 529).cpu().numpy(), blocks, append=c0 > 0)
+        log(f"ggft records: {c0 + b}/{n}")
+    return paths
+
+
 def save_dataset(path, cubes, G, Gz):
     np.save(path + ".cubes.npy", cubes)
     np.save(path + ".G.npy", G)
@@ -467,6 +560,8 @@ if __name__ == "__main__":
     ap.add_argument("--face-sampling", default="uniform", choices=("uniform", "prob"),
                     help="PoissonPredictor: faces drawn uniformly or with their FDM face probability")
     ap.add_argument("--make-data", metavar="PATH", help="only build the FDM dataset and save it under PATH.*.npy")
+    ap.add_argument("--make-bin", metavar="DIR", help="only write DIR/poisson.bin and DIR/gradient.bin in the GGFT "
+                    "dataset format (the reference's training input)")
     ap.add_argument("--data", metavar="PATH", help="use a dataset saved by --make-data")
     ap.add_argument("--only", help="train this network only (needs --data)")
     ap.add_argument("--init", metavar="DRWNET", help="with --only: start from these weights")
@@ -476,7 +571,11 @@ if __name__ == "__main__":
     def _log(msg):
         print(msg, flush=True)
 
-    if args.make_data:
+    if args.make_bin:
+        t0 = time.time()
+        paths = ggft_bin_dataset(args.make_bin, args.n, args.seed, log=_log)
+        _log(f"GGFT dataset: {args.n} records in {time.time() - t0:.1f}s -> {paths[0]}, {paths[1]}")
+    elif args.make_data:
         t0 = time.time()
         save_dataset(args.make_data, *fdm_dataset(args.n, args.seed))
         _log(f"dataset: {args.n} cubes in {time.time() - t0:.1f}s -> {args.make_data}.*.npy")

@@ -7,6 +7,8 @@
 // float64 or float32).  A file that ends early is an error here (the
 // reference reads past the end silently).  Built and run by
 // tests/test_offline_readers.py; exits non-zero on the first failure.
+// With --read-bin it only reads a given float64 dataset file and writes what
+// it read (tests/test_ggft_bin.py).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -66,6 +68,16 @@ static void check_records(const std::vector<float>& d, const std::vector<float>&
 
 int main(int argc, char** argv)
 {
+    // readers_check --read-bin FILE SAMPLES STRUCTURE_SIZE OUT: read a float64
+    // dataset file and write its dielectric and target floats to OUT
+    if (argc == 6 && std::string(argv[1]) == "--read-bin") {
+        std::vector<float> d, g;
+        DNNSolver::readBinaryFile(argv[2], d, g, std::atoi(argv[3]), std::atoi(argv[4]), DataType::DOUBLE, false);
+        std::ofstream o(argv[5], std::ios::binary);
+        o.write(reinterpret_cast<const char*>(d.data()), d.size() * sizeof(float));
+        o.write(reinterpret_cast<const char*>(g.data()), g.size() * sizeof(float));
+        return o ? 0 : 1;
+    }
     const std::string dir = argc > 1 ? argv[1] : ".";
     // ---- structure text file
     {

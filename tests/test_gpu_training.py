@@ -53,3 +53,36 @@ def test_net_gradients_match_cpu(name):
     worst = max(float((got[n] - ref[n]).norm()) / max(float(ref[n].norm()), 1e-3 * total) for n in ref)
     print(f"{name}: worst gradient relative error {worst:.2e}", flush=True)
     assert worst < 1e-3
+
+
+def test_ggft_dataset_files(tmp_path):
+    """poisson.bin / gradient.bin written from the GPU FDM (the reference's
+    training input format): the records' permittivities are the oracle's
+    voxelization of the same seeded descriptions, structure blocks are
+    theirs, G^GFT sums to 1 and is non-negative, G^z sums to 0, and two
+    records of each file match the oracle's FP64 solve (tolerance 1e-6)."""
+    from deeprwcap_amd import structures as S
+    from deeprwcap_amd import training as T
+    from oracle import oracle as O
+    n, seed = 96, 4
+    print(f"ggft dataset files: {n} records", flush=True)
+    paths = T.ggft_bin_dataset(str(tmp_path), n, seed, chunk=64, log=lambda m: print(m, flush=True))
+    L = np.concatenate([S.layered_boxes(b, 23, seed=1_000_003 * (seed + 1) + c0, max_boxes=2)[0]
+                        for c0, b in ((0, 64), (64, 32))])
+    C = np.concatenate([S.layered_boxes(b, 23, seed=1_000_003 * (seed + 1) + c0, max_boxes=2)[1]
+                        for c0, b in ((0, 64), (64, 32))])
+    want, _ = O.voxelize(L, C, 23, normalise=False)
+    for code in (0, 1):
+        cubes, blocks, G = T.read_ggft_bin(paths[code])
+        assert len(cubes) == n
+        np.testing.assert_array_equal(cubes, want.astype(np.float32).astype(np.float64))
+        np.testing.assert_array_equal(blocks, T.description_blocks(L, C))
+        s = G.sum(axis=(1, 2))
+        if code == 0:
+            assert np.abs(s - 1).max() < 1e-4 and G.min() > -1e-7
+        else:
+            assert np.abs(s).max() < 1e-4
+        for b in (0, n - 1):
+            ref, _, _ = O.fdm_solve(cubes[b], O.GFT if code == 0 else O.WVTZ, tol=1e-12)
+            err = np.abs(G[b].ravel() - ref.ravel()).max() / np.abs(ref).max()
+            assert err < 1e-4, (code, b, err)
