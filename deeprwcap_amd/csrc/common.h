@@ -68,6 +68,29 @@ __device__ __forceinline__ double readlane_f64(double v, int l)
     return __builtin_bit_cast(double, r);
 }
 
+// A value every active lane holds alike, moved to scalar registers (so the
+// compiler neither keeps a VGPR copy per lane nor spills one).
+__device__ __forceinline__ float uniform_f32(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+template <typename T>
+__device__ __forceinline__ T* uniform_ptr(T* p)
+{
+    const uint64_t u = (uint64_t)p;
+    return (T*)((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u) |
+                ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32));
+}
+
+__device__ __forceinline__ double uniform_f64(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint64_t r = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u) |
+                       ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32)) << 32);
+    return __builtin_bit_cast(double, r);
+}
+
 // Sum over the wave; every lane receives the total (the same bits in all).
 __device__ __forceinline__ float wave_sum(float v)
 {

@@ -7,27 +7,29 @@
 // (ggft/src/ggft_solver.cpp:447-518, 552-596).  The right-hand sides follow
 // update_ek (ggft_solver.cpp:137-274) for odd and even N.
 //
-// Design (one persistent 1024-thread workgroup per CU, one cube at a time):
-//  * cells c = tid + q*1024 (q < 12 at N = 23); the x and y face couplings
-//    of each cell, symmetrically Jacobi-scaled (w_cn / sqrt(d_c d_n)), live
-//    in VGPRs for the whole solve; the z couplings live in LDS (the z- one of
+// Design (persistent workgroups, one cube at a time each; 1024 threads and one
+// workgroup per CU up to N = 23, 256 threads and three per CU up to N = 16):
+//  * cells c = tid + q*T (q < 12 at N = 23); the x and y face couplings of
+//    each cell, symmetrically Jacobi-scaled (w_cn / sqrt(d_c d_n)), live in
+//    VGPRs for the whole solve; the z couplings live in LDS (the z- one of
 //    cell c is the z+ one of c - N^2), next to the search direction p
 //    (N^3 + 2 N^2 floats with zero guard planes, so the stencil reads need no
-//    bounds checks: x/y wrap-around reads hit a zero weight); A p is parked
-//    in a third LDS array across the reduction so the per-thread state stays
-//    at 4 couplings + x + r per cell: the CG iteration loop runs without
-//    spills at 12 cells per thread (the set-up and FP64 residual passes
-//    spill a few registers to scratch).  The Chronopoulos-Gear form, one
-//    fused reduction per iteration, needs x, p and A p per cell besides r
-//    and spills inside the loop at 12 cells per thread, with no LDS left to
-//    park a vector in;
+//    bounds checks: x/y wrap-around reads hit a zero weight) and the Jacobi
+//    scales s_c; A p, x and r of the own cells stay in registers;
 //  * inner solver: FP32 CG on the scaled system (= Jacobi PCG), block-wide
-//    reductions by wave64 butterflies + a 16-entry LDS table, summed in a
-//    fixed order so every thread sees bit-identical alpha/beta;
+//    reductions by wave64 DPP sums + a 16-entry LDS table, summed in a
+//    fixed order so every thread sees bit-identical alpha/beta (held in
+//    scalar registers);
 //  * outer loop: FP64 iterative refinement.  The residual e - L y is formed
-//    in FP64 with exact harmonic means, y accumulates in FP64, and the loop
-//    stops at ||e - L y|| <= tol ||e|| (the reference's relative-residual
-//    criterion, ggft_solver.h:123-126, 193-198);
+//    in FP64 with exact harmonic means and the loop stops at ||e - L y|| <=
+//    tol ||e|| (the reference's relative-residual criterion,
+//    ggft_solver.h:123-126, 193-198).  After the first FP32 solve y = ||e||
+//    S x is kept implicit (s and x in LDS, exact as a double product): the
+//    FP64 check and the G panels are computed from it, so a cube that
+//    converges in one solve (most of them, with the first solve aimed at
+//    0.3 tol) reads its permittivities and writes its 6 N^2 outputs and
+//    little else to HBM; a cube that needs a second round spills y to an
+//    FP64 workspace and refines from there;
 //  * work distribution: a self-resetting atomic ticket counter hands cubes to
 //    workgroups in the (optional) host-given order, e.g. largest N first.
 #include "common.h"
@@ -41,29 +43,26 @@ namespace drw {
 // 156 KB of LDS, one per CU; cubes up to N = 16 take 256 threads and 53 KB,
 // so three share a CU and each hides the others' barrier and LDS latency
 // (one cube per CU leaves most of it waiting at the block reductions).
-template <int T, int MAXN, int PER_CU, bool FUSED>
+template <int T, int MAXN, int PER_CU>
 struct FdmSharedT {
     static constexpr int kThreads = T, kWaves = T / 64, kMaxN = MAXN, kPerCU = PER_CU;
-    static constexpr bool kFusedCG = FUSED;  // Chronopoulos-Gear inner loop (else textbook CG)
     static constexpr int kGuard = MAXN * MAXN, kCells = MAXN * MAXN * MAXN;
     float p[kCells + 2 * kGuard + T];  // slack covers the unused tail slots
     float wz[kCells + kGuard];         // scaled z+ couplings, zero guard plane below
-    float ap[kCells];                  // A p of the current iteration (own cells only)
+    float ap[kCells];                  // eps while setting up, then the Jacobi scales s_c
     float red[2][kWaves];
-    float2 red2[kWaves];
     double redd[kWaves];
     double rhs_val[8];
     int rhs_idx[8];
     double enorm;
     int ticket;
 };
-// The large class keeps textbook CG: the Chronopoulos-Gear loop needs one
-// more vector in registers (12 VGPRs at N = 23), which spills at 1024 threads
-// per workgroup (N = 23 batch 16.1 ms against 12.5 ms); the small class has
-// the registers and gains from the saved barrier (N = 16 batch 3.35 ms
-// against 3.45 ms).
-using FdmLarge = FdmSharedT<1024, kMaxN, 1, false>;
-using FdmSmall = FdmSharedT<256, 16, 3, true>;  // registers capped so that three fit a CU
+// Both classes run textbook CG with A p in registers.  The Chronopoulos-Gear
+// form (one fused reduction per iteration) needs s = A p, x and p per cell
+// besides r and spills at these register budgets: N = 23 batch 16.1 ms
+// against 12.5 ms, N = 16 batch 3.96 ms against 2.95 ms.
+using FdmLarge = FdmSharedT<1024, kMaxN, 1>;
+using FdmSmall = FdmSharedT<256, 16, 3>;  // registers capped so that three fit a CU
 
 typedef __attribute__((address_space(3))) float lds_float;
 
@@ -85,26 +84,7 @@ __device__ __forceinline__ float block_sum(float v, float* red)
     float s = 0.f;
 #pragma unroll
     for (int w = 0; w < W; w++) s += red[w];
-    return s;
-}
-
-// Two sums with one barrier (the fused reduction of the Chronopoulos-Gear
-// loop).  The table is rewritten only after the caller's next barrier.
-template <int W>
-__device__ __forceinline__ float2 block_sum2(float a, float b, float2* red)
-{
-    a = wave_sum(a);
-    b = wave_sum(b);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = make_float2(a, b);
-    __syncthreads();
-    float2 s = make_float2(0.f, 0.f);
-#pragma unroll
-    for (int w = 0; w < W; w++) {
-        const float2 t = red[w];
-        s.x += t.x;
-        s.y += t.y;
-    }
-    return s;
+    return uniform_f32(s);
 }
 
 template <int W>
@@ -117,7 +97,7 @@ __device__ __forceinline__ double block_sum_d(double v, double* red)
 #pragma unroll
     for (int w = 0; w < W; w++) s += red[w];
     __syncthreads();  // table free for the next call
-    return s;
+    return uniform_f64(s);
 }
 
 // Harmonic-mean face permittivity 2ab/(a+b).  The FP32 form feeds only the
@@ -203,16 +183,26 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     constexpr int NN = N * N, NNN = N * NN;
     constexpr int T = SH::kThreads, W = SH::kWaves;
     constexpr int CPT = (NNN + T - 1) / T;
-    const int tid = threadIdx.x;
+    // A p stays in registers and the Jacobi scales s_c in the third LDS
+    // array for the whole solve (s is read before and after each inner solve
+    // only, so in registers it would be spilled across the loop).
+    // Opaque to the optimiser: the per-thread cell indices, LDS addresses and
+    // boundary masks derive from tid, so they are recomputed for each cube
+    // (a few VALU instructions) instead of being hoisted out of the
+    // persistent ticket loop for every cell and cube size and spilled to
+    // scratch, whose per-cube reloads (about 1 MB per workgroup) do not stay
+    // in L2.
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const float* E = a.E;
-    double* Y = a.Y;
     const int code = a.code;
     float* P = sh.p + NN;  // valid for indices [-NN, NNN + NN)
+    float* SC = sh.ap;     // s_c after the set-up
 
-    // ---- stage eps (in the A p array), clear guards and y, build the source ----
+    // ---- stage eps (in the A p array), clear guards, build the source -------
     float* WZ = sh.wz + NN;  // valid for indices [-NN, NNN)
-    float* EL = sh.ap;       // eps staging during set-up and residual passes
-    for (int c = tid; c < NNN; c += T) EL[c] = E[c];  // y starts at 0: the first update writes it
+    float* EL = sh.ap;       // eps staging during the set-up
+    for (int c = tid; c < NNN; c += T) EL[c] = E[c];
     for (int g = tid; g < NN; g += T) { P[g - NN] = 0.f; P[NNN + g] = 0.f; WZ[g - NN] = 0.f; }
     if (tid == 0) {
         fdm_rhs<N>(E, code, sh.rhs_idx, sh.rhs_val);
@@ -223,8 +213,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     __syncthreads();
 
     // ---- Jacobi scaling s_c = d_c^-1/2, d_c = sum_n w_cn + 2 eps_c nb(c) -----
-    // (own cells in registers for the whole solve, all cells in P for the
-    // coupling set-up below)
+    // (all cells in P for the coupling set-up below)
     float sc[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
@@ -265,42 +254,95 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
         }
     }
     __syncthreads();
+    // eps staging is done: s_c moves to the third array for good
+    for (int c = tid; c < NNN; c += T) SC[c] = P[c];
+    __syncthreads();
 
-    const double enorm = sh.enorm;
+    const double enorm = uniform_f64(sh.enorm);
     float* R32 = P;  // residual hand-off reuses the p array (p is rebuilt from it)
     double rnorm = enorm;
     int total_it = 0;
     bool have_y = false;  // y = 0 until the first update writes it
     bool converged = false;
+    // After the first FP32 solve y = ||e|| S x exactly (a double product of
+    // the FP32 s_c and x_c), so y stays implicit: s (SV) and x (XV, in the p
+    // array) of every cell sit in LDS, the FP64 check reads them there and G
+    // comes from them.  Only a cube that needs a second round writes y to Y.
+    const float* SV = SC;
+    float* XV = P;
+    bool implicit = false;
+    double yscale = 0.0;
     for (int round = 0; round < a.max_rounds; round++) {
-        // ---- FP64 residual r = e - L y, handed to the inner solve as FP32 S r ----
-        // (one pass per cell, not unrolled, through the A p array)
-        if (round > 0) {
-            for (int c = tid; c < NNN; c += T) EL[c] = E[c];
-            __syncthreads();
-        }
-        double loc = 0.0;
+        if (implicit) {
+            // ---- FP64 residual norm of y = yscale S x --------------------------
+            double loc = 0.0;
+            const float* Ev = E;  // address arithmetic kept inside the pass (not held, and spilled, across rounds)
+            asm volatile("" : "+s"(Ev));
 #pragma unroll 1
-        for (int c = tid; c < NNN; c += T) {
-            double rv = 0.0;
-            if (round > 0) {
+            for (int c = tid; c < NNN; c += T) {
                 const int i = c % N, j = (c / N) % N, k = c / NN;
-                const double ec = EL[c], yc = Y[c];
+                const double ec = Ev[c], uc = (double)SV[c] * (double)XV[c];
+                auto u = [&](int n) { return (double)SV[n] * (double)XV[n]; };
                 double acc = 0.0;
                 int nb = 0;
-                if (i > 0) acc += harm(ec, (double)EL[c - 1]) * (yc - Y[c - 1]); else nb++;
-                if (i < N - 1) acc += harm(ec, (double)EL[c + 1]) * (yc - Y[c + 1]); else nb++;
-                if (j > 0) acc += harm(ec, (double)EL[c - N]) * (yc - Y[c - N]); else nb++;
-                if (j < N - 1) acc += harm(ec, (double)EL[c + N]) * (yc - Y[c + N]); else nb++;
-                if (k > 0) acc += harm(ec, (double)EL[c - NN]) * (yc - Y[c - NN]); else nb++;
-                if (k < N - 1) acc += harm(ec, (double)EL[c + NN]) * (yc - Y[c + NN]); else nb++;
-                rv = -(acc + 2.0 * ec * nb * yc);
-            }
+                if (i > 0) acc += harm(ec, (double)Ev[c - 1]) * (uc - u(c - 1)); else nb++;
+                if (i < N - 1) acc += harm(ec, (double)Ev[c + 1]) * (uc - u(c + 1)); else nb++;
+                if (j > 0) acc += harm(ec, (double)Ev[c - N]) * (uc - u(c - N)); else nb++;
+                if (j < N - 1) acc += harm(ec, (double)Ev[c + N]) * (uc - u(c + N)); else nb++;
+                if (k > 0) acc += harm(ec, (double)Ev[c - NN]) * (uc - u(c - NN)); else nb++;
+                if (k < N - 1) acc += harm(ec, (double)Ev[c + NN]) * (uc - u(c + NN)); else nb++;
+                double rv = -yscale * (acc + 2.0 * ec * nb * uc);
 #pragma unroll
-            for (int s = 0; s < 8; s++)
-                if (sh.rhs_idx[s] == c) rv += sh.rhs_val[s];
-            loc += rv * rv;
-            R32[c] = (float)rv;  // scaled by s_c where the inner solve reads it
+                for (int s = 0; s < 8; s++)
+                    if (sh.rhs_idx[s] == c) rv += sh.rhs_val[s];
+                loc += rv * rv;
+            }
+            rnorm = sqrt(block_sum_d<W>(loc, sh.redd));  // also orders the LDS reads before reuse
+            if (!(rnorm > a.tol * enorm)) {
+                converged = true;
+                break;
+            }
+            // a further round: y goes to Y, and the residual is formed from it
+            double* Yv = a.Y;
+            asm volatile("" : "+s"(Yv));  // addresses of the rare path stay in it
+            for (int c = tid; c < NNN; c += T) Yv[c] = (double)SV[c] * (double)XV[c] * yscale;
+            implicit = false;
+            __syncthreads();
+        }
+        // ---- FP64 residual r = e - L y, handed to the inner solve as FP32 r ----
+        double loc = 0.0;
+        if (round == 0) {
+            for (int c = tid; c < NNN; c += T) {
+                double rv = 0.0;
+#pragma unroll
+                for (int s = 0; s < 8; s++)
+                    if (sh.rhs_idx[s] == c) rv += sh.rhs_val[s];
+                loc += rv * rv;
+                R32[c] = (float)rv;
+            }
+        } else {
+            const double* Yv = a.Y;
+            const float* Ev = E;
+            asm volatile("" : "+s"(Yv), "+s"(Ev));
+#pragma unroll 1
+            for (int c = tid; c < NNN; c += T) {
+                const int i = c % N, j = (c / N) % N, k = c / NN;
+                const double ec = Ev[c], yc = Yv[c];
+                double acc = 0.0;
+                int nb = 0;
+                if (i > 0) acc += harm(ec, (double)Ev[c - 1]) * (yc - Yv[c - 1]); else nb++;
+                if (i < N - 1) acc += harm(ec, (double)Ev[c + 1]) * (yc - Yv[c + 1]); else nb++;
+                if (j > 0) acc += harm(ec, (double)Ev[c - N]) * (yc - Yv[c - N]); else nb++;
+                if (j < N - 1) acc += harm(ec, (double)Ev[c + N]) * (yc - Yv[c + N]); else nb++;
+                if (k > 0) acc += harm(ec, (double)Ev[c - NN]) * (yc - Yv[c - NN]); else nb++;
+                if (k < N - 1) acc += harm(ec, (double)Ev[c + NN]) * (yc - Yv[c + NN]); else nb++;
+                double rv = -(acc + 2.0 * ec * nb * yc);
+#pragma unroll
+                for (int s = 0; s < 8; s++)
+                    if (sh.rhs_idx[s] == c) rv += sh.rhs_val[s];
+                loc += rv * rv;
+                R32[c] = (float)rv;  // scaled by s_c where the inner solve reads it
+            }
         }
         rnorm = sqrt(block_sum_d<W>(loc, sh.redd));  // also orders the R32 writes
         if (!(rnorm > a.tol * enorm)) {           // converged (or enorm == 0)
@@ -315,84 +357,17 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
         const float tgt2 = (float)(tgt * tgt);
         float xh[CPT];
         // Per-thread LDS bases; element q*T + k of each is cell c = tid + q*T
-        // shifted by k - N^2 (P, WZ) or by k (AP).
+        // shifted by k - N^2.
         lds_float* Pb = lds_base(sh.p + tid);
         lds_float* Zb = lds_base(sh.wz + tid);
-        lds_float* Ab = lds_base(sh.ap + tid);
-        if constexpr (SH::kFusedCG) {
-            // Chronopoulos-Gear CG (one fused reduction per iteration): r lives
-            // in P (the SpMV operand), p in AP, s = A p and x in registers.
-            // Per iteration: w = A r; (r, r) and (w, r) in one reduction;
-            // beta = g / g_prev, alpha = g / (d - beta g / alpha_prev);
-            // p = r + beta p, s = w + beta s, x += alpha p, r -= alpha s; one
-            // barrier makes the new r visible to the next SpMV.  Convergence
-            // is seen one SpMV later than in textbook CG (the reduction that
-            // measures r_k also runs the k-th SpMV).
-#pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
-                xh[q] = 0.f;
-                if (q < CPT - 1 || c < NNN) P[c] = R32[c] * sc[q] * inv;  // R32 aliases P: in place
-            }
-            __syncthreads();
-            float sv[CPT];
-            float g_prev = 1.f, a_prev = 1.f, thresh = 0.f;
-            for (int it = 0; it < a.max_inner; it++) {
-                float wv[CPT];
-                float gl = 0.f, dl = 0.f;
-#pragma unroll
-                for (int q = 0; q < CPT; q++) {
-                    const int c = tid + q * T;
-                    const int o = q * T + NN;  // P index of cell c relative to Pb
-                    wv[q] = 0.f;
-                    if (q < CPT - 1 || c < NNN) {
-                        const float rc = Pb[o];
-                        float v = rc;
-                        v -= w[0][q] * Pb[o - 1];
-                        v -= w[1][q] * Pb[o + 1];
-                        v -= w[2][q] * Pb[o - N];
-                        v -= w[3][q] * Pb[o + N];
-                        v -= Zb[q * T] * Pb[o - NN];
-                        v -= Zb[q * T + NN] * Pb[o + NN];
-                        wv[q] = v;
-                        gl += rc * rc;
-                        dl += v * rc;
-                    }
-                }
-                const float2 gd = block_sum2<W>(gl, dl, sh.red2);
-                const float g = gd.x;
-                if (it == 0) thresh = tgt2 * g;
-                else if (!(g > thresh)) break;
-                const float beta = it == 0 ? 0.f : g / g_prev;
-                const float den = gd.y - beta * g / a_prev;  // (p, A p) in exact arithmetic
-                if (!(den > 0.f)) break;                     // r is zero to rounding
-                const float alpha = g / den;
-#pragma unroll
-                for (int q = 0; q < CPT; q++) {
-                    const int c = tid + q * T;
-                    const int o = q * T + NN;
-                    if (q < CPT - 1 || c < NNN) {
-                        const float rc = Pb[o];
-                        const float pn = it == 0 ? rc : rc + beta * Ab[q * T];
-                        sv[q] = it == 0 ? wv[q] : wv[q] + beta * sv[q];
-                        Ab[q * T] = pn;
-                        xh[q] += alpha * pn;
-                        Pb[o] = rc - alpha * sv[q];
-                    }
-                }
-                g_prev = g;
-                a_prev = alpha;
-                total_it++;
-                __syncthreads();
-            }
-        } else {
+        {
             float rh[CPT];
             float rloc = 0.f;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
                 const int c = tid + q * T;
                 xh[q] = 0.f;
-                rh[q] = (q < CPT - 1 || c < NNN) ? R32[c] * sc[q] * inv : 0.f;
+                rh[q] = (q < CPT - 1 || c < NNN) ? R32[c] * SC[c] * inv : 0.f;
                 rloc += rh[q] * rh[q];
             }
             float rr = block_sum<W>(rloc, sh.red[1]);
@@ -405,10 +380,12 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             __syncthreads();
             for (int it = 0; it < a.max_inner; it++) {
                 float pl = 0.f;
+                float apv[CPT];
 #pragma unroll
                 for (int q = 0; q < CPT; q++) {
                     const int c = tid + q * T;
                     const int o = q * T + NN;  // P index of cell c relative to Pb
+                    apv[q] = 0.f;
                     if (q < CPT - 1 || c < NNN) {
                         const float pc = Pb[o];
                         float v = pc;
@@ -418,7 +395,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                         v -= w[3][q] * Pb[o + N];
                         v -= Zb[q * T] * Pb[o - NN];
                         v -= Zb[q * T + NN] * Pb[o + NN];
-                        Ab[q * T] = v;
+                        apv[q] = v;
                         pl += pc * v;
                     }
                 }
@@ -430,7 +407,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                     const int c = tid + q * T;
                     if (q < CPT - 1 || c < NNN) {
                         xh[q] += alpha * Pb[q * T + NN];
-                        rh[q] -= alpha * Ab[q * T];
+                        rh[q] -= alpha * apv[q];
                         rl += rh[q] * rh[q];
                     }
                 }
@@ -447,11 +424,25 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 __syncthreads();
             }
         }
-        // ---- y += ||r|| S x -----------------------------------------------------
+        // ---- y += ||r|| S x (implicit after the first solve) -------------------
+        // every thread is past its last read of its neighbours' p (the solve's
+        // final reduction), so P can take x
+        if (!have_y) {
+            implicit = true;
+            yscale = rnorm;
 #pragma unroll
-        for (int q = 0; q < CPT; q++) {
-            const int c = tid + q * T;
-            if (q < CPT - 1 || c < NNN) Y[c] = (have_y ? Y[c] : 0.0) + (double)sc[q] * (double)xh[q] * rnorm;
+            for (int q = 0; q < CPT; q++) {
+                const int c = tid + q * T;
+                if (q < CPT - 1 || c < NNN) XV[c] = xh[q];
+            }
+        } else {
+            double* Yv = a.Y;
+            asm volatile("" : "+s"(Yv));
+#pragma unroll
+            for (int q = 0; q < CPT; q++) {
+                const int c = tid + q * T;
+                if (q < CPT - 1 || c < NNN) Yv[c] += (double)SC[c] * (double)xh[q] * rnorm;
+            }
         }
         have_y = true;
         __syncthreads();
@@ -459,18 +450,40 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
 
     // ---- G_b = 2 eps_c y_c, GGFT panel order (ggft_solver.h:65-77) ----------
     double* G = a.G;
-    for (int pb = tid; pb < 6 * NN; pb += T) {
-        const int f = pb / NN, t = pb - f * NN, u = t % N, v = t / N;
-        int c;
-        switch (f) {
-        case 0: c = u + v * N; break;
-        case 1: c = u + v * N + (N - 1) * NN; break;
-        case 2: c = u + v * NN; break;
-        case 3: c = u + (N - 1) * N + v * NN; break;
-        case 4: c = u * N + v * NN; break;
-        default: c = (N - 1) + u * N + v * NN; break;
+    const float* Ev = E;
+    asm volatile("" : "+s"(G), "+s"(Ev));
+    if (implicit) {
+        // each boundary cell's owner writes its panels (a cell on an edge or a
+        // corner lies on two or three)
+#pragma unroll 1
+        for (int c = tid; c < NNN; c += T) {
+            const int i = c % N, j = (c / N) % N, k = c / NN;
+            if (i == 0 || i == N - 1 || j == 0 || j == N - 1 || k == 0 || k == N - 1) {
+                const double g = 2.0 * (double)Ev[c] * ((double)SV[c] * (double)XV[c] * yscale);
+                if (k == 0) G[i + j * N] = g;
+                if (k == N - 1) G[NN + i + j * N] = g;
+                if (j == 0) G[2 * NN + i + k * N] = g;
+                if (j == N - 1) G[3 * NN + i + k * N] = g;
+                if (i == 0) G[4 * NN + j + k * N] = g;
+                if (i == N - 1) G[5 * NN + j + k * N] = g;
+            }
         }
-        G[pb] = have_y ? 2.0 * (double)E[c] * Y[c] : 0.0;
+    } else {
+        const double* Yv = a.Y;
+        asm volatile("" : "+s"(Yv));
+        for (int pb = tid; pb < 6 * NN; pb += T) {
+            const int f = pb / NN, t = pb - f * NN, u = t % N, v = t / N;
+            int c;
+            switch (f) {
+            case 0: c = u + v * N; break;
+            case 1: c = u + v * N + (N - 1) * NN; break;
+            case 2: c = u + v * NN; break;
+            case 3: c = u + (N - 1) * N + v * NN; break;
+            case 4: c = u * N + v * NN; break;
+            default: c = (N - 1) + u * N + v * NN; break;
+            }
+            G[pb] = have_y ? 2.0 * (double)Ev[c] * Yv[c] : 0.0;
+        }
     }
     if (tid == 0) {
         if (a.iters) *a.iters = total_it;
@@ -483,9 +496,9 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
 // The job of cube b of a batch, solved by the calling workgroup.
 __device__ __forceinline__ CubeJob batch_job(const FdmBatch& a, int b, int cells)
 {
-    CubeJob j;
-    j.E = a.eps + a.eps_off[b];
-    j.G = a.G + a.G_off[b];
+    CubeJob j;  // the cube's pointers are the same in every thread: scalar registers
+    j.E = uniform_ptr(a.eps + a.eps_off[b]);
+    j.G = uniform_ptr(a.G + a.G_off[b]);
     j.Y = a.y_ws + (size_t)blockIdx.x * cells;  // fdm_launch sizes the grids to fit the workspace
     j.code = a.code[b];
     j.tol = a.tol;
