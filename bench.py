@@ -441,6 +441,9 @@ def main():
     ap.add_argument("--no-fdm-walks", action="store_true", help="--cap3d: network queries only")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (PMC byte-counter calibration)")
+    ap.add_argument("--fdm-profile", action="store_true",
+                    help="only solve the benchmark batch (8192 N=23 cubes, GFT, tol 1e-6) twice on the FDM path "
+                         "(for rocprofv3 kernel-trace and counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -453,6 +456,18 @@ def main():
     from deeprwcap_amd.dnnsolver import DNNSolverPoisson
 
     torch.cuda.set_device(0)
+    if args.fdm_profile:
+        from deeprwcap_amd import device as D
+        B = args.batch
+        L, C, _ = S.layered_boxes(B, 23, seed=0)
+        pk = D.PackedCubes([23] * B)
+        D.voxelize(L, C, fixed_N=23, normalise=False, out=pk.eps.view(B, 23, 23, 23))
+        for _ in range(2):
+            iters, relres = D.fdm_solve(pk, 0, tol=1e-6)
+        torch.cuda.synchronize()
+        log(f"fdm profile: {B} N=23 cubes x2, {float(iters.float().mean()):.1f} iterations, "
+            f"relres max {float(relres.max()):.2e}")
+        return
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
         dst = torch.empty_like(src)

@@ -133,15 +133,16 @@ __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float
 //  * border pixels: a tap outside the image reads `zero` (a zeroed LDS slot)
 //    instead, chosen per tap by a select, so the nine loads of an item still
 //    issue back to back.
-// Both passes sum the taps in the same order.  Needs 256 % (C/2) == 0 (a
-// thread's channel pair, and so its 18 weights, stays fixed); other channel
-// counts use the sliding form below.
+// Both passes sum the taps in the same order.  A thread's channel pair, and
+// so its 18 weights, stays fixed: items advance by the largest multiple of
+// C/2 up to 256 (252 threads at 24 channels, the other 4 idle here).
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
 __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, const float* w, const float* bias,
                                const TI* zero, const float2* gtab)
 {
     constexpr int CP = C / 2;
-    static_assert(256 % CP == 0, "channel pair fixed per thread");
+    constexpr int STRIDE = (256 / CP) * CP;
+    if (threadIdx.x >= STRIDE) return;
     constexpr int NI = kN - 2 * DIL;                 // interior side
     constexpr int NB = kNN - NI * NI;                // border pixels
     constexpr int NTOP = DIL * kN;                   // pixels in the top (and in the bottom) DIL rows
@@ -163,7 +164,7 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
         // three adjacent pixels per item: their taps are the 3 x 5 window
         // around them (15 loads instead of 27), the index costs once per three
         constexpr int G = 3, NG = NI / G;
-        for (int item = threadIdx.x; item < CP * NI * NG; item += 256) {
+        for (int item = threadIdx.x; item < CP * NI * NG; item += STRIDE) {
             const int q = item / CP;
             const int iy = q / NG;
             const int p = (iy + 1) * kN + 1 + G * (q - iy * NG);  // first pixel of the group
@@ -190,7 +191,7 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
         }
     } else {
 #pragma unroll 2
-        for (int item = threadIdx.x; item < CP * NI * NI; item += 256) {
+        for (int item = threadIdx.x; item < CP * NI * NI; item += STRIDE) {
             const int q = item / CP;
             const int iy = q / NI;
             const int p = q + iy * (kN - NI) + DIL * (kN + 1);  // (iy + DIL) kN + (q - iy NI) + DIL
@@ -205,7 +206,7 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
             finish(p, v0, v1);
         }
     }
-    for (int item = threadIdx.x; item < CP * NB; item += 256) {
+    for (int item = threadIdx.x; item < CP * NB; item += STRIDE) {
         const int q = item / CP;
         // top DIL rows, bottom DIL rows, then the DIL left and DIL right
         // pixels of each middle row
@@ -235,63 +236,12 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
     }
 }
 
-// Depthwise 3x3 conv (zero padding = dilation) + folded BN + GELU on the
-// VALU: in [544][CSI] -> out [544][CSO], channels 0..C-1, sliding form: a
-// thread owns one (channel pair, column x) and slides down the rows in steps
-// of the dilation, so each output reads 3 new input pairs (the other 6 taps
-// are reused from the previous step) and the column's x-border mask is
-// computed once.
-template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
-__device__ void dw_conv_slide(const TI* in, TO* out, const float* w, const float* bias, const float2* gtab)
-{
-    constexpr int CP = C / 2;
-    for (int item = threadIdx.x; item < CP * kN; item += 256) {
-        const int cp = item % CP, x = item / CP;
-        float w0[9], w1[9];
-#pragma unroll
-        for (int k = 0; k < 9; k++) { w0[k] = w[(2 * cp) * 9 + k]; w1[k] = w[(2 * cp + 1) * 9 + k]; }
-        const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
-        const bool okl = x - DIL >= 0, okr = x + DIL < kN;
-        // three input columns (x-d, x, x+d) of one row, both channels
-        auto load_row = [&](int y, float* v) {
-#pragma unroll
-            for (int q = 0; q < 6; q++) v[q] = 0.f;
-            if (y < 0 || y >= kN) return;
-            const TI* s = in + (y * kN + x) * CSI + 2 * cp;
-            load_pair(s, v[2], v[3]);
-            if (okl) load_pair(s - DIL * CSI, v[0], v[1]);
-            if (okr) load_pair(s + DIL * CSI, v[4], v[5]);
-        };
-#pragma unroll 1
-        for (int r0 = 0; r0 < DIL; r0++) {
-            float up[6], mid[6], dn[6];
-            load_row(r0 - DIL, up);
-            load_row(r0, mid);
-#pragma unroll 1
-            for (int y = r0; y < kN; y += DIL) {
-                load_row(y + DIL, dn);
-                float a0 = b0, a1 = b1;
-#pragma unroll
-                for (int kx = 0; kx < 3; kx++) {
-                    a0 += w0[kx] * up[2 * kx] + w0[3 + kx] * mid[2 * kx] + w0[6 + kx] * dn[2 * kx];
-                    a1 += w1[kx] * up[2 * kx + 1] + w1[3 + kx] * mid[2 * kx + 1] + w1[6 + kx] * dn[2 * kx + 1];
-                }
-                store_pair(out + (y * kN + x) * CSO + 2 * cp, gelu_tab(a0, gtab), gelu_tab(a1, gtab));
-#pragma unroll
-                for (int q = 0; q < 6; q++) { up[q] = mid[q]; mid[q] = dn[q]; }
-            }
-        }
-    }
-}
-
-// Depthwise layer: direct form when the channel pair can stay fixed per
-// thread, sliding form otherwise (24 channels).
+// Depthwise layer (the direct form for every channel count).
 template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
 __device__ __forceinline__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias, const float* zero,
                                         const float2* gtab)
 {
-    if constexpr (256 % (C / 2) == 0) dw_conv_direct<C, DIL, CSI, CSO>(in, out, w, bias, (const TI*)zero, gtab);
-    else dw_conv_slide<C, DIL, CSI, CSO>(in, out, w, bias, gtab);
+    dw_conv_direct<C, DIL, CSI, CSO>(in, out, w, bias, (const TI*)zero, gtab);
 }
 
 // Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4), fp32
