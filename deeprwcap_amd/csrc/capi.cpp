@@ -142,6 +142,7 @@ int drw_solver_load_models(drw_solver* s, const char* models_txt)
         (rc = s->fb_count.ensure(1)) || (rc = s->fb_ctr.ensure(2)))
         return rc;
     DRW_HIP(hipMemset(s->fb_ctr.p, 0, sizeof(uint32_t) * s->fb_ctr.n));
+    DRW_HIP(hipMemset(s->fb_count.p, 0, sizeof(int32_t)));  // the face kernels append from zero
     DRW_HIP(hipDeviceSynchronize());
     return 0;
 }
@@ -265,6 +266,8 @@ static int solver_run(drw_solver* s, int B, const float* L, const float* C, cons
         q.qid = dg.qid;
         q.ctr_dev = dg.ctr_dev;
         q.seed = s->seed; q.counter = s->counter; q.timer = &s->timer;
+        if (!s->fb_ctr.p) return set_error(DRW_ESTATE, "network fallback buffers missing: load the models first");
+        nn_fallback_attach(s, q);
         int rc = query_nn_launch(s->nets, q, &s->scratch.p, &s->scratch.n, st);
         // the FDM fallback of this batch, always enqueued: it reads the count of
         // failed rows on the device

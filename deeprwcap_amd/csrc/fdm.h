@@ -66,12 +66,12 @@ int fdm_accept_launch(const FdmAcceptArgs& a, hipStream_t stream);
 // Network-path fallback: the queries whose network result holds a NaN (a
 // failed draw: non-finite or all-zero kernel or probabilities) are solved by
 // the FDM at N = 23 and drawn with their own Philox counters, in one
-// persistent launch that exits at once when there are none.
-//   check: one workgroup scans out[B][width] and writes the rows with a NaN
-//     to list (count; stats[2] += count);
-//   solve: workgroups take listed queries by ticket, voxelize the
-//     description (raw permittivities), solve at tol (FP64 mode on a failed
-//     acceptance, as in the FDM path) and write the hop to out[q].
+// persistent launch that exits at once when there are none.  The face
+// kernels append those queries to list (count, stats[2]; stats[1] beyond
+// cap) as they write them; the fallback's workgroups take listed queries by
+// ticket, voxelize the description (raw permittivities), solve at tol (FP64
+// mode on a failed acceptance, as in the FDM path) and write the hop to
+// out[q]; the last one to finish zeroes count.
 struct NnFallbackArgs {
     int B, width, cap;           // queries in the batch, floats per result, list capacity
     float* out;                  // [B][width] network results, overwritten for listed queries
@@ -79,7 +79,7 @@ struct NnFallbackArgs {
     const float* cuboids;        // [B][15][7]
     const int32_t* axis;         // [B] (gradient) or null
     int32_t* list;               // [cap]
-    int32_t* count;              // [1]
+    int32_t* count;              // [1] appended by the face kernels, zeroed by the fallback
     uint64_t* stats;             // [0] FDM re-solves, [1] failed, [2] network fallbacks
     uint64_t seed, counter;      // Philox key and first query counter
     const uint64_t* qid;         // [B] per-query counters (optional)

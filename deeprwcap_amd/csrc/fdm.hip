@@ -644,29 +644,6 @@ int fdm_accept_launch(const FdmAcceptArgs& a, hipStream_t stream)
 
 // ------------------------------------------------------- network fallback
 // One workgroup: the rows of out[B][width] that hold a NaN, in list.
-__global__ __launch_bounds__(1024) void nn_check_kernel(NnFallbackArgs a)
-{
-    __shared__ int n;
-    if (threadIdx.x == 0) n = 0;
-    __syncthreads();
-    for (int b = threadIdx.x; b < a.B; b += blockDim.x) {
-        bool nan = false;
-        for (int j = 0; j < a.width; j++) nan |= isnan(a.out[(size_t)b * a.width + j]);
-        if (nan) {
-            const int k = atomicAdd(&n, 1);
-            if (k < a.cap) a.list[k] = b;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *a.count = min(n, a.cap);
-        if (n) {
-            atomicAdd((unsigned long long*)&a.stats[2], (unsigned long long)n);
-            if (n > a.cap) atomicAdd((unsigned long long*)&a.stats[1], (unsigned long long)(n - a.cap));
-        }
-    }
-}
-
 // Block-wide sum and minimum of one kernel (1024 threads), then acceptance.
 template <int W, typename SH>
 __device__ bool block_kernel_ok(const double* G, int M, int status, int code, SH& sh)
@@ -697,7 +674,7 @@ __global__ __launch_bounds__(FdmLarge::kThreads, 1) void nn_fallback_kernel(NnFa
     __shared__ FdmLarge sh;
     __shared__ float Ly[kMaxLayers * 2], Cb[kMaxCuboids * 7];
     constexpr int N = kMaxN, NN = N * N, NNN = N * NN;
-    const int n = *a.count;
+    const int n = min(*a.count, a.cap);  // appended by the face kernels of this batch
     const int slot = blockIdx.x;
     float* E = a.eps + (size_t)slot * kMaxCells;
     double* G = a.G + (size_t)slot * 6 * NN;
@@ -758,7 +735,8 @@ __global__ __launch_bounds__(FdmLarge::kThreads, 1) void nn_fallback_kernel(NnFa
     if (threadIdx.x == 0) {
         __threadfence();
         const uint32_t done = atomicAdd(&a.ctr[1], 1u);
-        if (done == gridDim.x - 1) {
+        if (done == gridDim.x - 1) {  // every workgroup has read the count: the list is free again
+            atomicExch(a.count, 0);
             atomicExch(&a.ctr[0], 0u);
             atomicExch(&a.ctr[1], 0u);
         }
@@ -769,8 +747,6 @@ int nn_fallback_launch(const NnFallbackArgs& a, hipStream_t stream)
 {
     if (a.B <= 0) return 0;
     if (a.grid <= 0 || a.cap <= 0) return set_error(-2, "nn_fallback_launch: no fallback buffers");
-    hipLaunchKernelGGL(nn_check_kernel, dim3(1), dim3(1024), 0, stream, a);
-    DRW_HIP(hipGetLastError());
     hipLaunchKernelGGL(nn_fallback_kernel, dim3(a.grid), dim3(FdmLarge::kThreads), 0, stream, a);
     DRW_HIP(hipGetLastError());
     return 0;

@@ -38,6 +38,10 @@ struct FaceQueryArgs {
     uint64_t seed, counter;
     const uint64_t* qid;     // [B] per-query counters (optional; else counter + b)
     const uint64_t* ctr_dev; // device-resident counter base (optional)
+    int32_t* fb_list;        // failed queries for the FDM fallback (optional)
+    int32_t* fb_count;
+    int fb_cap;
+    uint64_t* fb_stats;
 };
 
 // mode 0: PoissonPredictor, 1: Gradient1Predictor, 2: Gradient2Predictor.

@@ -475,6 +475,18 @@ __device__ __forceinline__ float block_sum256(float v, float* red)
 // ------------------------------------------------------------------ kernel
 // mode 0: Poisson predictor (relu head); 1: Gradient1 (norm head, faces 0,1);
 // 2: Gradient2 (FaceSolverPlus, norm head, faces 2..5).
+// A query whose result holds a NaN joins the FDM fallback's list (when the
+// caller runs the fallback): beyond the list's capacity it keeps its NaN and
+// counts as failed.
+__device__ __forceinline__ void fallback_append(const FaceQueryArgs& a, int b)
+{
+    if (!a.fb_count) return;
+    const int k = atomicAdd(a.fb_count, 1);
+    if (k < a.fb_cap) a.fb_list[k] = b;
+    else atomicAdd((unsigned long long*)&a.fb_stats[1], 1ull);
+    atomicAdd((unsigned long long*)&a.fb_stats[2], 1ull);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNetDev w)
 {
@@ -487,6 +499,7 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     if (face < 0) {  // failed face draw: a NaN result (the Poisson or the Gradient1 kernel writes it)
         const int width = MODE == 0 ? 3 : 5;
         if (MODE != 2 && threadIdx.x < width) a.out[(size_t)b * width + threadIdx.x] = __builtin_nanf("");
+        if (MODE != 2 && threadIdx.x == 0 && !a.full) fallback_append(a, b);
         return;
     }
     if (MODE == 1 && !(face == 0 || face == 1)) return;
@@ -545,22 +558,29 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
         if (threadIdx.x == 0) {
             const int width = MODE == 0 ? 3 : 5;
             float* o = a.out + (size_t)b * width;
+            float r[5];
             if (t < 0) {
-                for (int j = 0; j < width; j++) o[j] = __builtin_nanf("");
+                for (int j = 0; j < width; j++) r[j] = __builtin_nanf("");
             } else {
                 float pt[3];
                 panel_point(kN, t, MODE == 0 ? face : axis_face_remap(face, axis), pt);
-                o[0] = pt[0]; o[1] = pt[1]; o[2] = pt[2];
+                r[0] = pt[0]; r[1] = pt[1]; r[2] = pt[2];
                 if (MODE != 0) {
                     const float g = kern[t];
                     const float sg = g > 0.f ? 1.f : (g < 0.f ? -1.f : 0.f);
-                    o[3] = a.weight[b] * sg;
+                    r[3] = a.weight[b] * sg;
                     // centre of the axis-rotated cube times the max (dnn_solver_gradient.cpp:260-266)
                     int z = kN / 2, y = kN / 2, x = kN / 2;
                     remap_axis(axis, z, y, x);
-                    o[4] = vox_value(s.desc, z, y, x) * s.desc.maxv;
+                    r[4] = vox_value(s.desc, z, y, x) * s.desc.maxv;
                 }
             }
+            bool nan = false;
+            for (int j = 0; j < width; j++) {
+                o[j] = r[j];
+                nan |= isnan(r[j]);
+            }
+            if (nan) fallback_append(a, b);
         }
     }
 }

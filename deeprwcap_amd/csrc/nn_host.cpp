@@ -440,6 +440,7 @@ int query_nn_launch(const NetWeights* w, const QueryArgs& q, uint8_t** scratch, 
     fa.B = B; fa.full = q.full; fa.layers = q.layers; fa.cuboids = q.cuboids; fa.face = face;
     fa.axis = grad ? q.axis : nullptr; fa.weight = wgt; fa.out = q.out; fa.kern_out = q.kern_out;
     fa.seed = q.seed; fa.counter = q.counter; fa.qid = q.qid; fa.ctr_dev = q.ctr_dev;
+    fa.fb_list = q.fb_list; fa.fb_count = q.fb_count; fa.fb_cap = q.fb_cap; fa.fb_stats = q.fb_stats;
     if (!grad) return timed(2, [&] { return face_query_launch(0, fa, w->pred, stream); });
     rc = timed(2, [&] {
         int r = face_query_launch(1, fa, w->g1, stream);

@@ -100,6 +100,8 @@ extern "C" int drw_solver_sample_device_noalloc(drw_solver* s, int B, const floa
     q.seed = s->seed;
     q.counter = s->counter;
     q.timer = nullptr;  // stage events are created on demand: not here
+    if (!s->fb_ctr.p) return set_error(DRW_ESTATE, "network fallback buffers missing: load the models first");
+    nn_fallback_attach(s, q);
     uint8_t* p = buf;
     const hipStream_t st = stream ? (hipStream_t)stream : s->stream;
     int rc = query_nn_launch(s->nets, q, &p, &cap, st);

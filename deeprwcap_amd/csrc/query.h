@@ -31,6 +31,11 @@ struct QueryArgs {
     const uint64_t* qid;    // [B] per-query Philox counters (optional; else counter + b)
     const uint64_t* ctr_dev;  // device-resident counter base (optional; replaces counter)
     StageTimer* timer;      // optional
+    // FDM fallback of failed rows (optional: set when nn_fallback_launch follows)
+    int32_t* fb_list;       // [fb_cap] failed queries, appended by the face kernels
+    int32_t* fb_count;      // [1] zero at rest; the fallback kernel resets it
+    int fb_cap;
+    uint64_t* fb_stats;     // [1] failed beyond the list, [2] network fallbacks
 };
 
 // Enqueues the fused network query; *scratch (grown as needed, *cap bytes)

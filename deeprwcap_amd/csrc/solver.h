@@ -77,6 +77,16 @@ namespace drw {
 constexpr int kFallbackSlots = 32;     // workgroups (and buffer slots) of the network-path fallback
 constexpr int kFallbackCap = 16384;    // failed network queries per batch that the fallback takes
 
+// Lets the face kernels of q append their failed queries to the fallback's
+// list; nn_fallback_enqueue must follow query_nn_launch(q).
+inline void nn_fallback_attach(drw_solver* s, QueryArgs& q)
+{
+    q.fb_list = s->fb_list.p;
+    q.fb_count = s->fb_count.p;
+    q.fb_cap = kFallbackCap;
+    q.fb_stats = s->stats;
+}
+
 // Enqueues the network-path FDM fallback for the batch q just enqueued
 // (queries whose result holds a NaN are solved by the FDM and drawn with their
 // own counters); allocates nothing.
