@@ -393,6 +393,33 @@ def cap3d_check(models_txt, path, walks, fdm=True, threads=32):
     return out
 
 
+def device_mix_ceiling(models_txt, B=2048, reps=10):
+    """Queries/s of the serving mix (N = 23, half Poisson, half gradient) on
+    the device alone: one batch of B of each kind back to back on resident
+    descriptions, the ceiling the server's batches of up to B can reach."""
+    import torch
+    from deeprwcap_amd import structures as S
+    from deeprwcap_amd.dnnsolver import DNNSolverGrad, DNNSolverPoisson
+    L, C, _ = S.layered_boxes(B, 23, seed=99)
+    dL, dC = torch.from_numpy(L).cuda(), torch.from_numpy(C).cuda()
+    dA = torch.from_numpy((np.arange(B) % 3).astype(np.int32)).cuda()
+    out = torch.empty((B, 5), device="cuda")
+    t = {}
+    for kind, cls in ((0, DNNSolverPoisson), (1, DNNSolverGrad)):
+        sol = cls(0)
+        sol.loadModels(models_txt=models_txt)
+        for _ in range(2):
+            sol.sample_device(B, dL, dC, dA if kind else None, out)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            sol.sample_device(B, dL, dC, dA if kind else None, out)
+        torch.cuda.synchronize()
+        t[kind] = (time.perf_counter() - t0) / reps
+    return {"batch": B, "poisson_ms": 1e3 * t[0], "gradient_ms": 1e3 * t[1],
+            "mix_queries_per_s": 2 * B / (t[0] + t[1])}
+
+
 def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False):
     """Serving loop: 64 host threads issue mixed Poisson/gradient queries with
     Poisson arrivals (N = 23, or N uniform over {16, 21, 23} with mixed_n);
@@ -405,7 +432,7 @@ def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False):
     L, C, _ = S.layered_boxes(pool, Ns, seed=99)
     kinds = rng.integers(0, 2, pool).astype(np.int32)
     axes = rng.integers(0, 3, pool).astype(np.int32)
-    sv = Server(0, models_txt, max_batch=1024, max_inflight=4, hbm_budget_mb=512)
+    sv = Server(0, models_txt, max_batch=2048, max_inflight=4, hbm_budget_mb=1024)
     sv.run_load(L, C, kinds, Ns, axes, threads=threads, rate_qps=0, duration_s=0.3)  # warm-up
     out = []
     for r in rates:
@@ -593,8 +620,11 @@ def main():
     if not args.no_mixed:
         res["mixed_walkers"] = mixed_walkers(models_txt)
     if not args.no_serving:
-        rates = [1e5, 1e6, 1e7, 0]
-        res["serving"] = serving_runs(models_txt, rates)
+        rates = [1e5, 1e6, 2e6, 3e6, 5e6, 1e7, 0]
+        res["serving_device_ceiling"] = device_mix_ceiling(models_txt)
+        log(f"serving mix on the device alone (batches of 2048): "
+            f"{res['serving_device_ceiling']['mix_queries_per_s']:,.0f}/s")
+        res["serving"] = serving_runs(models_txt, rates, duration_s=1.5)
         res["p99_latency_ms"] = {("back_to_back" if r["offered_qps"] is None else f"offered_{r['offered_qps']:.0e}"):
                                  r["p99_ms"] for r in res["serving"]}
         sustained = [r["offered_qps"] for r in res["serving"] if r["offered_qps"] and r["p99_ms"] < 1.0
