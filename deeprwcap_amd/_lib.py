@@ -33,6 +33,8 @@ SIGNATURES = {
     "drw_solver_output_width": (_c_int, [_c_vp]),
     "drw_solver_set_fdm_limits": (_c_int, [_c_vp, _c_int, _c_int]),
     "drw_solver_failures": (_c_int, [_c_vp, ctypes.POINTER(_c_u64), ctypes.POINTER(_c_u64), ctypes.POINTER(_c_u64)]),
+    "drw_set_launch_shape": (_c_int, [_c_int, _c_int, _c_int]),
+    "drw_get_launch_shape": (_c_int, [_c_int, _c_int]),
     "drw_solver_set_timing": (_c_int, [_c_vp, _c_int]),
     "drw_solver_stage_times": (_c_int, [_c_vp, _c_vp, _c_vp, _c_int]),
     "drw_solver_sample": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),

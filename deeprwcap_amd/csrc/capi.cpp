@@ -178,6 +178,19 @@ int drw_solver_get_counter(const drw_solver* s, uint64_t* counter)
 
 int drw_solver_output_width(const drw_solver* s) { return s ? (s->kind == DRW_GRADIENT ? 5 : 3) : -1; }
 
+int drw_set_launch_shape(int which, int min_items, int threads)
+{
+    clear_error();
+    return drw::set_launch_shape(which, min_items, threads);
+}
+
+int drw_get_launch_shape(int which, int items)
+{
+    clear_error();
+    const int t = drw::launch_shape(which, items);
+    return t > 0 ? t : drw::set_error(DRW_EINVAL, "launch shape %d: no such kernel", which);
+}
+
 int drw_solver_set_timing(drw_solver* s, int enable)
 {
     if (!s) return set_error(DRW_EINVAL, "set_timing: null solver");

@@ -18,6 +18,14 @@ enum SolveCode : int { kGFT = 0, kWVTZ = 1, kWVTX = 2, kWVTY = 3 };
 // Records the first error of the calling thread; returns the code.
 int set_error(int code, const char* fmt, ...);
 
+// Launch shapes chosen at run time, process-wide (drw_set_launch_shape):
+// threads per workgroup of the tunable kernels as rules (min_items,
+// threads); a launch of `items` workgroups takes the rule with the largest
+// min_items <= items, else 256 threads.
+enum LaunchShape : int { kShapeFacePoisson = 0, kShapeFaceGrad1 = 1, kShapeFaceGrad2 = 2, kShapeCount = 3 };
+int launch_shape(int which, int items);
+int set_launch_shape(int which, int min_items, int threads);
+
 #define DRW_HIP(call)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

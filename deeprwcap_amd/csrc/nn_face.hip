@@ -33,7 +33,7 @@ namespace drw {
 // alone the FaceSolverPlus kernels come within a few percent of the 1e-3 L1
 // bound against the fp32 forward.  Both parts accumulate in one chain that
 // starts from the bias, so the epilogue is the GELU alone.
-template <int CIN_PAD, int COUT, int COUT_PAD, int CSI, int CSO>
+template <int NTH, int CIN_PAD, int COUT, int COUT_PAD, int CSI, int CSO>
 __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, const half4_t* frag_lo, const float* bias,
                         const float2* gtab)
 {
@@ -52,7 +52,7 @@ __device__ void pw_mfma(const half_t* in, half_t* out, const half4_t* frag, cons
 #pragma unroll
         for (int i = 0; i < 4; i++) bb[n][i] = n * 16 + kq + i < COUT ? bias[n * 16 + kq + i] : 0.f;
     }
-    for (int m = wave; m < kPix / 16; m += 4) {
+    for (int m = wave; m < kPix / 16; m += NTH / 64) {
         half4_t a[KT];
 #pragma unroll
         for (int k = 0; k < KT; k++) a[k] = *(const half4_t*)(in + (m * 16 + r) * CSI + k * 16 + kq);
@@ -136,12 +136,12 @@ __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float
 // Both passes sum the taps in the same order.  A thread's channel pair, and
 // so its 18 weights, stays fixed: items advance by the largest multiple of
 // C/2 up to 256 (252 threads at 24 channels, the other 4 idle here).
-template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
+template <int NTH, int C, int DIL, int CSI, int CSO, typename TI, typename TO>
 __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, const float* w, const float* bias,
                                const TI* zero, const float2* gtab)
 {
     constexpr int CP = C / 2;
-    constexpr int STRIDE = (256 / CP) * CP;
+    constexpr int STRIDE = (NTH / CP) * CP;
     if (threadIdx.x >= STRIDE) return;
     constexpr int NI = kN - 2 * DIL;                 // interior side
     constexpr int NB = kNN - NI * NI;                // border pixels
@@ -237,19 +237,19 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
 }
 
 // Depthwise layer (the direct form for every channel count).
-template <int C, int DIL, int CSI, int CSO, typename TI, typename TO>
+template <int NTH, int C, int DIL, int CSI, int CSO, typename TI, typename TO>
 __device__ __forceinline__ void dw_conv(const TI* in, TO* out, const float* w, const float* bias, const float* zero,
                                         const float2* gtab)
 {
-    dw_conv_direct<C, DIL, CSI, CSO>(in, out, w, bias, (const TI*)zero, gtab);
+    dw_conv_direct<NTH, C, DIL, CSI, CSO>(in, out, w, bias, (const TI*)zero, gtab);
 }
 
 // Small 1x1 conv + folded BN + GELU on the VALU (CIN <= 8, COUT <= 4), fp32
 // activations.
-template <int CIN, int COUT, int CSI, int CSO>
+template <int NTH, int CIN, int COUT, int CSI, int CSO>
 __device__ void pw_valu(const float* in, float* out, const float* w, const float* bias, const float2* gtab)
 {
-    for (int p = threadIdx.x; p < kNN; p += 256) {
+    for (int p = threadIdx.x; p < kNN; p += NTH) {
         float v[CIN];
 #pragma unroll
         for (int c = 0; c < CIN; c++) v[c] = (float)in[p * CSI + c];
@@ -264,10 +264,10 @@ __device__ void pw_valu(const float* in, float* out, const float* w, const float
 }
 
 // Head 1x1 conv CIN -> 1 with bias, no activation: fp32 per pixel.
-template <int CIN, int CSI>
+template <int NTH, int CIN, int CSI>
 __device__ void head_conv(const float* in, float* hout, const float* w, float b)
 {
-    for (int p = threadIdx.x; p < kNN; p += 256) {
+    for (int p = threadIdx.x; p < kNN; p += NTH) {
         float a = b;
 #pragma unroll
         for (int c = 0; c < CIN; c++) a += w[c] * in[p * CSI + c];
@@ -303,7 +303,7 @@ __device__ __forceinline__ void input_src(int face, int axis, int c, int h, int 
 // The product runs in split precision, W a ~= Wh ah + Wh al + Wl ah with
 // fp16 parts of the fp32 operands: the input layer's rounding is otherwise
 // the largest share of the network's error against the fp32 forward.
-template <int MODE, int COUT, int CSO>
+template <int NTH, int MODE, int COUT, int CSO>
 __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out, const half4_t* frag,
                               const half4_t* frag_lo, const float* bias, const float2* gtab)
 {
@@ -349,7 +349,7 @@ __device__ void pw_input_mfma(const VoxDesc& d, int face, int axis, half_t* out,
     // slice) and a fixed one (layer value per pixel).
     auto tiles = [&](auto zmov_c) {
         constexpr bool ZM = decltype(zmov_c)::value;
-        for (int m = wave; m < kPix / 16; m += 4) {
+        for (int m = wave; m < kPix / 16; m += NTH / 64) {
             const int p = m * 16 + r;
             const int pc = min(p, kNN - 1);
             const int h = pc / kN, w = pc - h * kN;
@@ -422,52 +422,56 @@ struct FaceSmem {
 };
 
 // Head output: float[529] in Y.
-template <int MODE>
+template <int NTH, int MODE>
 __device__ __forceinline__ void face_solver(FaceSmem<16>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.frag_lo[0], w.pw_bias[0], s.gtab); __syncthreads();
-    dw_conv<16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
-    dw_conv<16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
+    pw_input_mfma<NTH, MODE, 16, 16>(s.desc, face, axis, s.X, w.frag[0], w.frag_lo[0], w.pw_bias[0], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 1, 16, 16>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 1, 16, 16>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 8, 16, 16, 16>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
     // fp32 tail
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
-    dw_conv<8, 2, 16, 8>(s.X, fY, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
-    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[3], w.pw_bias[3], s.gtab); __syncthreads();
-    dw_conv<4, 3, 4, 4>(fX, fY, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
-    pw_valu<4, 2, 4, 2>(fY, fX, w.pw_w[4], w.pw_bias[4], s.gtab); __syncthreads();
-    head_conv<2, 2>(fX, fY, w.head_w, w.head_b);
+    dw_conv<NTH, 8, 2, 16, 8>(s.X, fY, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
+    pw_valu<NTH, 8, 4, 8, 4>(fY, fX, w.pw_w[3], w.pw_bias[3], s.gtab); __syncthreads();
+    dw_conv<NTH, 4, 3, 4, 4>(fX, fY, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
+    pw_valu<NTH, 4, 2, 4, 2>(fY, fX, w.pw_w[4], w.pw_bias[4], s.gtab); __syncthreads();
+    head_conv<NTH, 2, 2>(fX, fY, w.head_w, w.head_b);
 }
 
+template <int NTH>
 __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetDev& w, int face, int axis)
 {
-    pw_input_mfma<2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.frag_lo[0], w.pw_bias[0], s.gtab); __syncthreads();
-    dw_conv<32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
-    dw_conv<32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
-    dw_conv<24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.frag_lo[3], w.pw_bias[3], s.gtab); __syncthreads();
-    dw_conv<24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
-    pw_mfma<32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
-    dw_conv<16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
-    dw_conv<16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
-    pw_mfma<16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
+    pw_input_mfma<NTH, 2, 32, 32>(s.desc, face, axis, s.X, w.frag[0], w.frag_lo[0], w.pw_bias[0], s.gtab); __syncthreads();
+    dw_conv<NTH, 32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
+    dw_conv<NTH, 32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
+    dw_conv<NTH, 24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.frag_lo[3], w.pw_bias[3], s.gtab); __syncthreads();
+    dw_conv<NTH, 24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
-    dw_conv<8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
-    pw_valu<8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7], s.gtab); __syncthreads();
-    head_conv<4, 4>(fX, fY, w.head_w, w.head_b);
+    dw_conv<NTH, 8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
+    pw_valu<NTH, 8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7], s.gtab); __syncthreads();
+    head_conv<NTH, 4, 4>(fX, fY, w.head_w, w.head_b);
 }
 
-__device__ __forceinline__ float block_sum256(float v, float* red)
+template <int NTH>
+__device__ __forceinline__ float block_sum(float v, float* red)
 {
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    const float s = red[0] + red[1] + red[2] + red[3];
+    float s = red[0];
+#pragma unroll
+    for (int i = 1; i < NTH / 64; i++) s += red[i];
     __syncthreads();
     return s;
 }
@@ -487,8 +491,8 @@ __device__ __forceinline__ void fallback_append(const FaceQueryArgs& a, int b)
     atomicAdd((unsigned long long*)&a.fb_stats[2], 1ull);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNetDev w)
+template <int MODE, int NTH>
+__global__ __launch_bounds__(NTH) void face_query_kernel(FaceQueryArgs a, FaceNetDev w)
 {
     constexpr int BCS = MODE == 2 ? 32 : 16;
     __shared__ FaceSmem<BCS> s;
@@ -506,21 +510,21 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     if (MODE == 2 && !(face >= 2 && face <= 5)) return;
     if (threadIdx.x < 64) vox_desc_build(s.desc, a.layers, a.cuboids, b);
     if (threadIdx.x < 4) s.zero[threadIdx.x] = 0.f;
-    for (int i = threadIdx.x; i < kGeluTabEntries; i += 256) s.gtab[i] = w.gtab[i];
+    for (int i = threadIdx.x; i < kGeluTabEntries; i += NTH) s.gtab[i] = w.gtab[i];
     __syncthreads();
-    if constexpr (MODE == 2) face_solver_plus(s, w, face, axis); else face_solver<MODE>(s, w, face, axis);
+    if constexpr (MODE == 2) face_solver_plus<NTH>(s, w, face, axis); else face_solver<NTH, MODE>(s, w, face, axis);
     __syncthreads();
     const float* hout = (const float*)s.Y;
     // ---- output head: relu + 1e-10 then /sum, or x / sum|x| (architecture.py:122-132)
     float loc = 0.f;
-    for (int p = threadIdx.x; p < kNN; p += 256) {
+    for (int p = threadIdx.x; p < kNN; p += NTH) {
         const float v = hout[p];
         loc += MODE == 0 ? (fmaxf(v, 0.f) + 1e-10f) : fabsf(v);
     }
-    const float tot = block_sum256(loc, s.red);
+    const float tot = block_sum<NTH>(loc, s.red);
     // normalised kernel -> buffer X as fp32 [529], post-processed for gradients
     float* kern = (float*)s.X;
-    for (int p = threadIdx.x; p < kNN; p += 256) {
+    for (int p = threadIdx.x; p < kNN; p += NTH) {
         float v;
         if (MODE == 0) {
             v = (fmaxf(hout[p], 0.f) + 1e-10f) / tot;
@@ -542,12 +546,12 @@ __global__ __launch_bounds__(256) void face_query_kernel(FaceQueryArgs a, FaceNe
     __syncthreads();
     if (a.full) {
         float* o = a.kern_out + (size_t)item * kNN;
-        for (int p = threadIdx.x; p < kNN; p += 256) o[p] = kern[p];
+        for (int p = threadIdx.x; p < kNN; p += NTH) o[p] = kern[p];
         return;
     }
     if (a.kern_out) {
         float* o = a.kern_out + (size_t)b * kNN;
-        for (int p = threadIdx.x; p < kNN; p += 256) o[p] = kern[p];
+        for (int p = threadIdx.x; p < kNN; p += NTH) o[p] = kern[p];
     }
     // ---- tile draw with the query's second uniform word, then locate
     if (threadIdx.x < 64) {
@@ -589,9 +593,18 @@ int face_query_launch(int mode, const FaceQueryArgs& a, const FaceNetDev& w, hip
 {
     const int items = a.full ? a.B * 6 : a.B;
     if (items <= 0) return 0;
-    if (mode == 0) hipLaunchKernelGGL(face_query_kernel<0>, dim3(items), dim3(256), 0, stream, a, w);
-    else if (mode == 1) hipLaunchKernelGGL(face_query_kernel<1>, dim3(items), dim3(256), 0, stream, a, w);
-    else hipLaunchKernelGGL(face_query_kernel<2>, dim3(items), dim3(256), 0, stream, a, w);
+    // threads per (query, face): the launch shape of this network's kernel
+    const bool wide = launch_shape(mode, items) == 512;
+    if (mode == 0) {
+        if (wide) hipLaunchKernelGGL((face_query_kernel<0, 512>), dim3(items), dim3(512), 0, stream, a, w);
+        else hipLaunchKernelGGL((face_query_kernel<0, 256>), dim3(items), dim3(256), 0, stream, a, w);
+    } else if (mode == 1) {
+        if (wide) hipLaunchKernelGGL((face_query_kernel<1, 512>), dim3(items), dim3(512), 0, stream, a, w);
+        else hipLaunchKernelGGL((face_query_kernel<1, 256>), dim3(items), dim3(256), 0, stream, a, w);
+    } else {
+        if (wide) hipLaunchKernelGGL((face_query_kernel<2, 512>), dim3(items), dim3(512), 0, stream, a, w);
+        else hipLaunchKernelGGL((face_query_kernel<2, 256>), dim3(items), dim3(256), 0, stream, a, w);
+    }
     DRW_HIP(hipGetLastError());
     return 0;
 }

@@ -51,10 +51,13 @@ class Server:
 
     @classmethod
     def from_tuning(cls, path, device=0, models_txt=None, **kw):
-        """Server with the batch shape chosen by deeprwcap_amd.autotune."""
+        """Server with the batch shape chosen by deeprwcap_amd.autotune (whose
+        launch shapes are applied to the whole process)."""
         import json
+        from . import autotune
         with open(path) as f:
             t = json.load(f)
+        autotune.apply(t)
         kw.setdefault("max_batch", int(t["max_batch"]))
         kw.setdefault("max_inflight", int(t.get("max_inflight", 4)))
         return cls(device, models_txt, **kw)

@@ -128,6 +128,22 @@ int drw_solver_output_width(const drw_solver* s);
 int drw_solver_set_timing(drw_solver* s, int enable);
 int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int reset);
 
+/* Launch shapes (no counterpart in the reference, whose TensorRT engines fix
+ * theirs at build time): threads per workgroup of the tunable kernels,
+ * process-wide and read at every launch, so they apply to every solver and
+ * server.  Per kernel, rules (min_items, threads): a launch of `items`
+ * workgroups uses the rule with the largest min_items <= items, else 256
+ * threads.  set with min_items >= 0 adds or replaces that rule (threads 0
+ * removes it; at most 8 rules); with min_items < 0 it makes `threads` the
+ * shape of every size, or with threads 0 restores the built-in rules (the
+ * autotuner's choice on MI355X).  get returns the threads a launch of
+ * `items` workgroups uses.  Chosen per machine by the autotuner
+ * (deeprwcap_amd.autotune).  DRW_SHAPE_FACE_* are the face predictors'
+ * kernels (one workgroup per query and face, 256 or 512 threads). */
+enum drw_shape { DRW_SHAPE_FACE_POISSON = 0, DRW_SHAPE_FACE_GRAD1 = 1, DRW_SHAPE_FACE_GRAD2 = 2 };
+int drw_set_launch_shape(int which, int min_items, int threads);
+int drw_get_launch_shape(int which, int items);
+
 /* sample(layers, cuboids[, axis]) with host pointers; blocks until `out`
  * holds B * width floats.  B = size(layers) / 30 in the reference
  * (dnn_solver_poisson.cpp:40).  Gradient solvers require axis (the reference

@@ -28,6 +28,19 @@ def params():
     return {n: R.params_from_npz(d, n) for n in R.ARCH}
 
 
+@pytest.fixture(params=[256, 512], ids=["256threads", "512threads"])
+def face_shape(request):
+    """Every face kernel at one launch shape (threads per query and face);
+    afterwards the library's built-in rules again."""
+    from deeprwcap_amd import _lib
+    lib = _lib.lib()
+    for w in range(3):
+        _lib.check(lib.drw_set_launch_shape(w, -1, request.param), "set_launch_shape")
+    yield request.param
+    for w in range(3):
+        _lib.check(lib.drw_set_launch_shape(w, -1, 0), "set_launch_shape")
+
+
 def _solver(kind, models_txt):
     from deeprwcap_amd.dnnsolver import DNNSolverGrad, DNNSolverPoisson
     s = (DNNSolverPoisson if kind == 0 else DNNSolverGrad)(0)
@@ -53,7 +66,7 @@ def _eval(s, L, C, axis):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_nn_kernels_match_fp32(kind, models_txt, params):
+def test_nn_kernels_match_fp32(kind, models_txt, params, face_shape):
     from deeprwcap_amd import structures as S
     from oracle import nets_ref as R
     B = 24
@@ -63,7 +76,7 @@ def test_nn_kernels_match_fp32(kind, models_txt, params):
     probs, wts, kern = _eval(s, L, C, axis)
     rp, rw, rk = R.query_kernels(kind, L, C, axis, params)
     l1 = np.abs(kern - rk).sum(-1)
-    print(f"kind {kind}: probs max err {np.abs(probs - rp).max():.2e}, kernel L1 max {l1.max():.2e} "
+    print(f"kind {kind}, {face_shape} threads: probs max err {np.abs(probs - rp).max():.2e}, kernel L1 max {l1.max():.2e} "
           f"mean {l1.mean():.2e}", flush=True)
     assert np.abs(probs - rp).max() < 2e-3
     assert l1.max() < L1_TOL
@@ -72,7 +85,7 @@ def test_nn_kernels_match_fp32(kind, models_txt, params):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_full_descriptions_match_fp32(kind, models_txt, params):
+def test_full_descriptions_match_fp32(kind, models_txt, params, face_shape):
     """Descriptions at the format's limits (15 layers, 15 overlapping
     cuboids, no padding) through the fused in-kernel voxelization."""
     from deeprwcap_amd import structures as S
@@ -93,7 +106,7 @@ def test_full_descriptions_match_fp32(kind, models_txt, params):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_trained_weights_match_fp32(kind):
+def test_trained_weights_match_fp32(kind, face_shape):
     """The same parity on the shipped weights (models/): trained kernels are
     peaked, so the fp16 activations' error shows far more than on seeded
     random weights, whose kernels are close to uniform."""
@@ -111,7 +124,7 @@ def test_trained_weights_match_fp32(kind):
     probs, wts, kern = _eval(s, L, C, axis)
     rp, rw, rk = R.query_kernels(kind, L, C, axis, params)
     l1 = np.abs(kern - rk).sum(-1)
-    print(f"trained kind {kind}: probs max err {np.abs(probs - rp).max():.2e}, kernel L1 max {l1.max():.2e} "
+    print(f"trained kind {kind}, {face_shape} threads: probs max err {np.abs(probs - rp).max():.2e}, kernel L1 max {l1.max():.2e} "
           f"mean {l1.mean():.2e}", flush=True)
     assert np.abs(probs - rp).max() < 2e-3
     assert l1.max() < L1_TOL
@@ -120,7 +133,7 @@ def test_trained_weights_match_fp32(kind):
 
 
 @pytest.mark.parametrize("kind", [0, 1])
-def test_nn_hops_bitexact(kind, models_txt, orc):
+def test_nn_hops_bitexact(kind, models_txt, orc, face_shape):
     import torch
     from deeprwcap_amd import _lib, structures as S
     B = 64
