@@ -314,6 +314,13 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
             avg.update_parameters(model)
         return loss
 
+    best, best_state = float("inf"), None
+    if init:
+        # a warm restart keeps its starting weights unless training beats them
+        # on the held-out cubes
+        best = held_out()
+        best_state = {k: v.detach().clone() for k, v in model.state_dict().items()}
+        log(f"{name}: starting weights held-out L1 {best:.4f}")
     model.train()
     t0 = time.time()
     if minutes is not None:
@@ -335,7 +342,6 @@ def train_net(name, cubes, G, Gz, epochs=30, batch=256, lr=1e-3, dev="cuda", log
         log(f"{name}: {per * 1e3:.1f} ms/step at batch {batch} -> {total} steps")
     else:
         total = epochs * max(1, (ntr * len(faces)) // batch)
-    best, best_state = float("inf"), None
     tot_l, nb = 0.0, 0
     for st in range(total):
         loss = step(st, total)
