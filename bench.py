@@ -420,10 +420,12 @@ def device_mix_ceiling(models_txt, B=2048, reps=10):
             "mix_queries_per_s": 2 * B / (t[0] + t[1])}
 
 
-def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False):
+def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False, resident=False):
     """Serving loop: 64 host threads issue mixed Poisson/gradient queries with
     Poisson arrivals (N = 23, or N uniform over {16, 21, 23} with mixed_n);
-    returns per-rate latency and throughput."""
+    returns per-rate latency and throughput.  resident: N = 23 queries go to
+    the resident loop (persistent kernel on the pinned request ring) instead
+    of batches."""
     from deeprwcap_amd import structures as S
     from deeprwcap_amd.serving import Server
     pool = 4096
@@ -432,7 +434,7 @@ def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False):
     L, C, _ = S.layered_boxes(pool, Ns, seed=99)
     kinds = rng.integers(0, 2, pool).astype(np.int32)
     axes = rng.integers(0, 3, pool).astype(np.int32)
-    sv = Server(0, models_txt, max_batch=2048, max_inflight=4, hbm_budget_mb=1024)
+    sv = Server(0, models_txt, max_batch=2048, max_inflight=4, hbm_budget_mb=1024, resident=resident)
     sv.run_load(L, C, kinds, Ns, axes, threads=threads, rate_qps=0, duration_s=0.3)  # warm-up
     out = []
     for r in rates:
@@ -440,8 +442,8 @@ def serving_runs(models_txt, rates, duration_s=2.0, threads=64, mixed_n=False):
                          max_outstanding=256)
         out.append({"offered_qps": r if r > 0 else None, "achieved_qps": st["qps"], "p50_ms": st["p50_ms"],
                     "p99_ms": st["p99_ms"], "mean_batch": st["mean_batch"], "completed": st["completed"]})
-        log(f"serving offered {r:g}/s: {st['qps']:,.0f}/s p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms "
-            f"batch {st['mean_batch']:.1f}")
+        log(f"serving{' (resident)' if resident else ''} offered {r:g}/s: {st['qps']:,.0f}/s "
+            f"p50 {st['p50_ms']:.3f} ms p99 {st['p99_ms']:.3f} ms batch {st['mean_batch']:.1f}")
     sv.close()
     return out
 
@@ -457,6 +459,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving runs (batched and resident) on the shipped weights")
     ap.add_argument("--capacitance", type=int, default=0, metavar="WALKS",
                     help="only run the end-to-end check: capacitance row of the 2x8-wire structure from network "
                          "queries vs FDM-only queries (N = 23), WALKS walks each")
@@ -511,6 +515,14 @@ def main():
     if args.capacitance:
         models_txt, wkind = models_txt_or_random(args.models)
         print(json.dumps(capacitance_check(models_txt, args.capacitance)), flush=True)
+        return
+    if args.serving_only:
+        models_txt, wkind = models_txt_or_random(args.models)
+        rates = [1e5, 1e6, 2e6, 3e6, 5e6, 1e7, 0]
+        res = {"serving_device_ceiling": device_mix_ceiling(models_txt),
+               "serving": serving_runs(models_txt, rates, duration_s=1.5),
+               "serving_resident": serving_runs(models_txt, rates, duration_s=1.5, resident=True)}
+        print(json.dumps(res), flush=True)
         return
     B = args.batch
     L, C, _ = S.layered_boxes(B, 23, seed=0)
@@ -631,6 +643,13 @@ def main():
                      and r["achieved_qps"] >= 0.95 * r["offered_qps"]]
         res["max_offered_qps_with_p99_under_1ms"] = max(sustained) if sustained else None
         res["serving_mixed_n"] = serving_runs(models_txt, [1e5, 1e6], mixed_n=True)
+        res["serving_resident"] = serving_runs(models_txt, rates, duration_s=1.5, resident=True)
+        res["p99_latency_ms_resident"] = {
+            ("back_to_back" if r["offered_qps"] is None else f"offered_{r['offered_qps']:.0e}"): r["p99_ms"]
+            for r in res["serving_resident"]}
+        sustained = [r["offered_qps"] for r in res["serving_resident"] if r["offered_qps"] and r["p99_ms"] < 1.0
+                     and r["achieved_qps"] >= 0.95 * r["offered_qps"]]
+        res["max_offered_qps_with_p99_under_1ms_resident"] = max(sustained) if sustained else None
     if not args.no_cpu:
         log("timing the CPU path ...")
         res["cpu_baseline"] = cpu_baseline(L, C, models_txt)

@@ -53,6 +53,7 @@ SIGNATURES = {
     "drw_server_submit_many": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_server_wait_many": (_c_int, [_c_vp, _c_int, _c_vp, _c_vp, _c_int]),
     "drw_server_stats_get": (_c_int, [_c_vp, _c_vp, _c_int]),
+    "drw_server_resident_stats": (_c_int, [_c_vp, _c_vp]),
     "drw_server_destroy": (None, [_c_vp]),
     "drw_walk_capacitance": (_c_int, [_c_vp, _c_vp, _c_int, _c_vp, _c_vp, _c_vp, _c_vp]),
     "drw_server_run_load": (_c_int, [_c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_vp, _c_int, _c_vp]),

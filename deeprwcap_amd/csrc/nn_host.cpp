@@ -388,6 +388,10 @@ int net_weights_load(const char* models_txt, int kind, NetWeights** out)
     return 0;
 }
 
+const SelNetDev& net_selector(const NetWeights* w) { return w->sel; }
+
+const FaceNetDev& net_face(const NetWeights* w, int which) { return which == 0 ? w->pred : which == 1 ? w->g1 : w->g2; }
+
 void net_weights_free(NetWeights* w)
 {
     if (!w) return;

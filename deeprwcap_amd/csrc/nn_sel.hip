@@ -15,193 +15,10 @@
 #include "common.h"
 #include "nn_common.h"
 #include "nn_sel.h"
+#include "nn_sel_body.h"
 #include "sampling.h"
 
 namespace drw {
-
-// Conv3d(CIN -> COUT, k3, s2, no padding) over in[ci][z][y][x] (fp16,
-// strides SC, SZ, SY, 1) as an implicit GEMM (rows = output positions,
-// k = ci*27 + tap) with bias + GELU, into out[co][oz*DOUT^2+oy*DOUT+ox],
-// fp16 or fp32 (FP32_OUT); `out` must not overlap `in`.  koff (LDS,
-// >= KT*16 ints) receives the im2col offset of every k (-1 past K), so a
-// gathered element costs one add and one LDS read.  With COUT < 16 only COUT
-// of the 16 MFMA columns hold outputs; the valid ones are first moved across
-// lanes (ds_bpermute) so that every lane evaluates 16*COUT/64 GELUs instead
-// of 4.  Ends with a barrier: outputs visible, koff free for the next layer.
-template <int CIN, int COUT, int DIN, int DOUT, int SC, int SZ, int SY, bool FP32_OUT>
-__device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, const float* bias, int* koff)
-{
-    constexpr int DO2 = DOUT * DOUT, M = DOUT * DO2;
-    constexpr int K = CIN * 27, KT = (K + 15) / 16;
-    constexpr int NT = (COUT + 15) / 16;
-    constexpr int VPT = COUT < 16 ? 16 * COUT / 64 : 4 * NT;  // outputs per lane per tile
-    constexpr int TP = (M + 15) / 16, TPW = (TP + 3) / 4;    // tiles / tiles per wave
-    static_assert(COUT >= 16 || (16 * COUT) % 64 == 0, "compacted tiles need COUT a multiple of 4");
-    for (int k = threadIdx.x; k < KT * 16; k += 256) {
-        int o = -1;
-        if (k < K) {
-            const int ci = k / 27, tap = k - 27 * ci;
-            o = ci * SC + (tap / 9) * SZ + ((tap / 3) % 3) * SY + tap % 3;
-        }
-        koff[k] = o;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = lane & 15, kq = 4 * (lane >> 4);
-    half4_t bw[NT][KT > 2 ? 1 : KT];  // fragments of short K kept in registers
-    if constexpr (KT <= 2) {
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int kt = 0; kt < KT; kt++) bw[n][kt] = frag[(n * KT + kt) * 64 + lane];
-    }
-#pragma unroll
-    for (int u = 0; u < TPW; u++) {
-        const int t = wave + 4 * u;
-        if (t >= TP) continue;
-        const int m = t * 16 + r;
-        int base = -1;
-        if (m < M) {
-            const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
-            base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
-        }
-        float4_t acc[NT];
-#pragma unroll
-        for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
-        for (int kt = 0; kt < KT; kt++) {
-            const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
-            const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
-            half4_t a;
-#pragma unroll
-            for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
-#pragma unroll
-            for (int n = 0; n < NT; n++) {
-                half4_t b;
-                if constexpr (KT <= 2) b = bw[n][kt];
-                else b = frag[(n * KT + kt) * 64 + lane];
-                acc[n] = mfma16(a, b, acc[n]);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < VPT; j++) {
-            float v;
-            int ch, mm;
-            if constexpr (COUT < 16) {
-                // output o = lane*VPT + j -> (row, col) = (o / COUT, o % COUT),
-                // held by lane col + 16*(row/4) as element row % 4
-                const int o = lane * VPT + j, row = o / COUT;
-                ch = o % COUT;
-                mm = t * 16 + row;
-                const int src = ch + 16 * (row >> 2), el = row & 3;
-                v = 0.f;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const float tv = __shfl(acc[0][i], src, 64);
-                    v = el == i ? tv : v;
-                }
-            } else {
-                ch = (j >> 2) * 16 + r;
-                mm = t * 16 + kq + (j & 3);
-                v = acc[j >> 2][j & 3];
-            }
-            const float g = gelu(v + bias[ch]);
-            if (mm < M) {
-                if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
-                else ((half_t*)outp)[ch * M + mm] = (half_t)g;
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// First Conv3d (1 -> C1, k3, s2, no padding) + folded BN + GELU (gelu_tab_sel) on the VALU,
-// one output z-slab per phase, in place: slab oz reads input planes
-// 2oz..2oz+2, so after the phase's barrier the planes below 2oz+2 are dead
-// and the slab's outputs (held in registers until then) are written over
-// them as out[oz][c][oy*11+ox] (C1 * 121 <= 2 * 529).  An item is one output
-// pixel of the slab and one channel pair; its 27 taps lie inside the cube
-// for every output, so they are LDS reads at fixed offsets from one base
-// address.  The slab base offset is even and the plane and row strides
-// (529, 23) are odd, so in tap row (tz, ty) the x-taps 0, 1 (tz + ty even)
-// or 1, 2 (odd) form a 4-byte-aligned pair: a row is one 32-bit and one
-// 16-bit read, and the products are v_dot2_f32_f16 (fp16 inputs and
-// weights, fp32 accumulate) over the nine pairs and the nine remaining taps
-// paired up in row order (conv1_pairs on the host lays the weights out the
-// same way).  With one input channel (K = 27) and 4 or 8 outputs, the
-// implicit-GEMM form spends most of its instructions on the gather and on
-// moving the valid columns out of the 16-wide MFMA tile.
-template <int C1>
-__device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, const float* bias, const float2* gtab)
-{
-    constexpr int DO = 11, DO2 = DO * DO, CP = C1 / 2, NP = 14;
-    constexpr int IPT = (DO2 * CP + 255) / 256;  // items per thread and phase
-    static_assert(C1 % 2 == 0 && 256 % CP == 0, "a thread's channel pair stays fixed");
-    static_assert(C1 * DO2 <= 2 * kNN, "slab outputs must fit the freed planes");
-    const int cp = threadIdx.x % CP;
-    half2_t w0[NP], w1[NP];
-#pragma unroll
-    for (int k = 0; k < NP; k++) {
-        w0[k] = ((const half2_t*)w)[(2 * cp) * NP + k];
-        w1[k] = ((const half2_t*)w)[(2 * cp + 1) * NP + k];
-    }
-    const float b0 = bias[2 * cp], b1 = bias[2 * cp + 1];
-    for (int oz = 0; oz < DO; oz++) {
-        float held[IPT][2];
-        int pix[IPT];
-#pragma unroll
-        for (int u = 0; u < IPT; u++) {
-            const int mm = (threadIdx.x + 256 * u) / CP;
-            pix[u] = mm;
-            float a0 = b0, a1 = b1;
-            if (mm < DO2) {
-                const int oy = mm / DO, ox = mm - oy * DO;
-                const half_t* src = cube + 2 * oz * kNN + 2 * oy * kN + 2 * ox;
-                half_t single[9];
-#pragma unroll
-                for (int r = 0; r < 9; r++) {
-                    const int tz = r / 3, ty = r % 3, par = (tz + ty) & 1;
-                    const half_t* row = src + tz * kNN + ty * kN;
-                    const half2_t pr = *(const half2_t*)(row + par);
-                    single[r] = row[par ? 0 : 2];
-                    a0 = __builtin_amdgcn_fdot2(pr, w0[r], a0, false);
-                    a1 = __builtin_amdgcn_fdot2(pr, w1[r], a1, false);
-                }
-#pragma unroll
-                for (int k = 0; k < 5; k++) {
-                    const half2_t sp = {single[2 * k], k < 4 ? single[2 * k + 1] : (half_t)0.f};
-                    a0 = __builtin_amdgcn_fdot2(sp, w0[9 + k], a0, false);
-                    a1 = __builtin_amdgcn_fdot2(sp, w1[9 + k], a1, false);
-                }
-            }
-            held[u][0] = gelu_tab_sel(a0, gtab);
-            held[u][1] = gelu_tab_sel(a1, gtab);
-        }
-        __syncthreads();  // this slab's input planes fully read
-        half_t* o = out + oz * (C1 * DO2);
-#pragma unroll
-        for (int u = 0; u < IPT; u++) {
-            if (pix[u] < DO2) {
-                o[(2 * cp) * DO2 + pix[u]] = (half_t)held[u][0];
-                o[(2 * cp + 1) * DO2 + pix[u]] = (half_t)held[u][1];
-            }
-        }
-    }
-    __syncthreads();  // outputs visible
-}
-
-// LDS of one query: the cube, the first conv's output (written slab by slab
-// over the cube planes it no longer needs, slab-major [oz][c][11*11]) and
-// the second conv's output ([c][125], after the first's) share one region:
-// 26.4 KB in all for FaceSelector (6 workgroups per CU), 27.4 KB for
-// FaceSelectorWeight (5 per CU).
-template <int C1, int C2>
-struct SelSmem {
-    static constexpr int kRegion = kNNN > C1 * 1331 + C2 * 125 ? kNNN : C1 * 1331 + C2 * 125;
-    alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4);
-                                    //           the first conv's GELU table before that
-    half_t cube[kRegion];           // >= 24,334 B
-    VoxDesc desc;
-};
 
 template <bool WEIGHTED>
 __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs a, SelNetDev w)
@@ -217,93 +34,8 @@ __global__ __launch_bounds__(256) void selector_backbone_kernel(SelBackboneArgs 
     else
         for (int i = threadIdx.x - 64; i < kSelGeluTab; i += 192) ((float2*)s.koff)[i] = w.gtab[i];
     __syncthreads();
-    const int axis = a.axis ? a.axis[b] : 2;
-    // Cube in the (axis pre-rotated) frame, built row by row: along an
-    // x-row of the output exactly one source axis varies (+-1 per voxel), so
-    // the cuboid masks of the two fixed source axes are AND-ed once per row
-    // chunk; items are quarter rows (6, 6, 6, 5 voxels) to balance the waves.
-    for (int it = threadIdx.x; it < kNN * 4; it += 256) {
-        const int r = it >> 2, q = it & 3;  // row r = 23 z + y of the output frame
-        const int zo = r / kN, yo = r - zo * kN;
-        // source of output voxel (zo, yo, x) under remap_axis: axis 0 ->
-        // (22 - x, yo, zo), axis 1 -> (22 - yo, zo, x), none / 2 -> (zo, yo, x)
-        const VoxDesc& d = s.desc;
-        const uint16_t* vm;
-        int base, dir, zfix;
-        uint32_t fixedm;
-        const bool zvar = a.axis && axis == 0;
-        if (zvar) {
-            vm = d.bz; base = kN - 1; dir = -1; zfix = 0;
-            fixedm = (uint32_t)d.by[yo] & d.bx[zo];
-        } else {
-            const bool ax1 = a.axis && axis == 1;
-            const int zs = ax1 ? kN - 1 - yo : zo, ys = ax1 ? zo : yo;
-            vm = d.bx; base = 0; dir = 1; zfix = zs;
-            fixedm = (uint32_t)d.bz[zs] & d.by[ys];
-        }
-        const float lfix = d.lz[zfix];
-        const int xa = 6 * q, xb = q == 3 ? kN : xa + 6;
-        half_t* dst = s.cube + r * kN;
-        for (int x = xa; x < xb; x++) {
-            const int k = base + dir * x;
-            const uint32_t m = fixedm & vm[k];
-            dst[x] = (half_t)(m ? d.cv[__builtin_ctz(m)] : (zvar ? d.lz[k] : lfix));
-        }
-    }
-    __syncthreads();
-    half_t* l1 = s.cube;                 // [11][C1][121], over the cube
-    half_t* l2 = s.cube + C1 * 1331;     // [C2][125], after l1
-    conv1_direct<C1>(s.cube, l1, w.conv1_w, w.bias[0], (const float2*)s.koff);
-    conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false>(l1, l2, w.frag[1], w.bias[1], s.koff);
-    conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, true>(l2, a.feat + (size_t)b * C3 * 8, w.frag[2], w.bias[2], s.koff);
-}
-
-// Linear layer on MFMA for 16 rows: X[16][K] (fp16, LDS) -> Y[16][N].
-template <int K, int N, bool ACT>
-__device__ void linear16(const half_t* X, half_t* Y, float* Yf, const half4_t* frag, const float* bias)
-{
-    constexpr int KT = K / 16, NT = (N + 15) / 16;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = lane & 15, kq = 4 * (lane >> 4);
-    for (int n = wave; n < NT; n += 4) {
-        float4_t acc = {0.f, 0.f, 0.f, 0.f};
-        for (int kt = 0; kt < KT; kt++) {
-            const half4_t av = *(const half4_t*)(X + r * K + kt * 16 + kq);
-            acc = mfma16(av, frag[(n * KT + kt) * 64 + lane], acc);
-        }
-        const int col = n * 16 + r;
-        if (col >= N) continue;
-        const float bb = bias[col];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = kq + i;
-            float v = acc[i] + bb;
-            if (ACT) v = gelu(v);
-            if (Yf) Yf[row * 16 + col] = v;
-            else Y[row * N + col] = (half_t)v;
-        }
-    }
-}
-
-// Face draw over six probabilities, sequentially (same arithmetic as the
-// wave-parallel draw: fixed-point quantisation + exact cumulative sums).
-__device__ int small_cdf_draw(const float* w, int M, uint32_t u)
-{
-    float m = 0.f;
-    for (int i = 0; i < M; i++) m = fmaxf(m, fabsf(w[i]));
-    if (!(m > 0.f) || isinf(m)) return -1;
-    int E;
-    frexpf(m, &E);
-    const int sh = 24 - E;
-    uint64_t total = 0;
-    for (int i = 0; i < M; i++) total += (uint32_t)ldexpf(fabsf(w[i]), sh);
-    const uint64_t r = (uint64_t)u * (total >> 32) + (((uint64_t)u * (total & 0xFFFFFFFFull)) >> 32);
-    uint64_t acc = 0;
-    for (int i = 0; i < M; i++) {
-        acc += (uint32_t)ldexpf(fabsf(w[i]), sh);
-        if (acc > r) return i;
-    }
-    return M - 1;
+    selector_backbone_body<WEIGHTED, 256>(s.desc, s.cube, s.koff, a.axis != nullptr, a.axis ? a.axis[b] : 2, w,
+                                          a.feat + (size_t)b * C3 * 8);
 }
 
 template <bool WEIGHTED>
@@ -336,22 +68,13 @@ __global__ __launch_bounds__(256) void selector_head_kernel(SelHeadArgs a, SelNe
     if (row < 16 && b0 + row < a.B) {
         const int b = b0 + row;
         const float* lg = logits + row * 16;
-        float mx = lg[0];
-        for (int i = 1; i < 6; i++) mx = fmaxf(mx, lg[i]);
-        float p[6], sum = 0.f;
-        for (int i = 0; i < 6; i++) { p[i] = expf(lg[i] - mx); sum += p[i]; }
-        for (int i = 0; i < 6; i++) p[i] /= sum;
+        const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
+        float p[6];
+        const int face = selector_probs_draw(lg, a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, p);
         if (a.probs)
             for (int i = 0; i < 6; i++) a.probs[(size_t)b * 6 + i] = p[i];
         if (WEIGHTED && a.weight) a.weight[b] = lg[6];
-        if (a.face) {
-            uint32_t u[4];
-            const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
-            query_uniforms(a.seed, a.qid ? a.qid[b] : base + (uint64_t)b, 0u, u);
-            // -1: no drawable face (non-finite or zero probabilities); the face
-            // kernel then writes a NaN result, which the FDM fallback picks up
-            a.face[b] = small_cdf_draw(p, 6, u[0]);
-        }
+        if (a.face) a.face[b] = face;
     }
 }
 

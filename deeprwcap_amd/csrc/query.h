@@ -2,6 +2,8 @@
 #pragma once
 
 #include "common.h"
+#include "nn_face.h"
+#include "nn_sel.h"
 #include "runtime.h"
 
 namespace drw {
@@ -14,6 +16,11 @@ struct NetWeights;  // device-resident packed weights of one solver kind
 // dnn_solver_gradient.cpp:19-22).
 int net_weights_load(const char* models_txt, int kind, NetWeights** out);
 void net_weights_free(NetWeights* w);
+
+// Device views of a loaded model set: its selector, and its predictors
+// (which: 0 PoissonPredictor, 1 Gradient1Predictor, 2 Gradient2Predictor).
+const SelNetDev& net_selector(const NetWeights* w);
+const FaceNetDev& net_face(const NetWeights* w, int which);
 
 struct QueryArgs {
     int B;

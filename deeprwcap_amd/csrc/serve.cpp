@@ -20,6 +20,14 @@
 // done -> free), the class queues, the completion wake-ups and the statistics
 // have separate locks, and callers that issue many queries at once (a walker
 // thread with many walks in flight) submit and wait for them in one call.
+//
+// Resident mode (cfg.resident): network queries (N = 23, both kinds) bypass
+// the batches.  submit() writes them into the pinned request ring of the
+// resident kernel (resident.h) and publishes them; the kernel, once
+// started by the dispatcher, serves them with no host step per query, and
+// waiters read the results from the response ring.  While FDM-route batches
+// are pending or running, the dispatcher stops the resident kernel (its
+// workgroups hold every CU) and starts it again afterwards.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +46,8 @@
 
 #include "../../include/dnnsolver_c.h"
 #include "common.h"
+#include "query.h"
+#include "resident.h"
 #include "runtime.h"
 
 using Clock = std::chrono::steady_clock;
@@ -58,6 +68,8 @@ struct Req {
     float in[kIn];
     float out[5];
     Clock::time_point t_submit;
+    uint64_t rseq = 0;  // resident-ring sequence
+    bool res = false;   // waiting for the resident kernel
 };
 
 struct Slot {
@@ -118,8 +130,37 @@ struct drw_server {
     uint64_t completed = 0, batches = 0, batched = 0, inflight_peak = 0;
     Clock::time_point t0 = Clock::now();
     std::thread worker;
+    // resident loop (cfg.resident)
+    bool resident = false;
+    drw::NetWeights* rw[2] = {nullptr, nullptr};
+    void* rnets = nullptr;          // device views of both model sets
+    drw::ResReq* rreq = nullptr;    // pinned request ring (host view)
+    drw::ResResp* rresp = nullptr;  // pinned response ring (host view)
+    uint32_t* rstop = nullptr;      // pinned stop flag
+    drw::ResCtl* rctl = nullptr;    // device control block
+    drw::ResidentArgs rargs{};
+    hipStream_t rstream = nullptr;
+    hipEvent_t rdone = nullptr;
+    int rgrid = 0;
+    std::mutex rmu;                       // sequence allocation and publication
+    uint64_t rnext = 0;                   // next sequence (under rmu)
+    std::atomic<uint64_t> rpub{0};        // sequences published
+    std::unique_ptr<std::atomic<uint64_t>[]> rfree;  // per slot: the sequence that may take it next
+    std::unique_ptr<std::atomic<Req*>[]> rowner;  // per slot: the request of its current sequence
+    uint64_t rscan = 0;                   // dispatcher: lowest sequence not yet reaped
+    std::vector<Req*> rdone_buf;          // dispatcher: requests completed in one reaper pass
+    std::atomic<bool> rrunning{false};    // resident kernel launched and not seen to end (dispatcher-owned)
+    std::atomic<uint64_t> rlaunches{0};
+    int rate_khz = 0;                     // device wall clock
+    Clock::duration ridle{};              // the dispatcher stops the kernel after this long idle
+    Clock::time_point rlast{};            // last resident submit or completion
 
     int slot_solver(Slot& s, int cls, drw_solver** out);
+    int res_create();
+    void res_destroy();
+    int res_launch();
+    void res_manage(int busy, int rout);
+    int res_reap();
     int launch(Slot& s, int cls, std::vector<Req*>& batch);
     void reap(Slot& s);
     void loop();
@@ -208,6 +249,114 @@ void drw_server::reap(Slot& s)
     s.busy = false;
 }
 
+// Resident loop: pinned rings, device control block, weights of both kinds.
+int drw_server::res_create()
+{
+    int rc = drw::net_weights_load(models.c_str(), 0, &rw[0]);
+    if (!rc) rc = drw::net_weights_load(models.c_str(), 1, &rw[1]);
+    if (rc) return rc;
+    const int threads = cfg.resident_threads ? cfg.resident_threads : 512;
+    if ((rc = drw::resident_grid(device, threads, &rgrid))) return rc;
+    if ((rc = drw::resident_nets_create(rw[0], rw[1], &rnets))) return rc;
+    DRW_HIP(hipHostMalloc((void**)&rreq, sizeof(drw::ResReq) * drw::kResRing, hipHostMallocMapped | hipHostMallocCoherent));
+    DRW_HIP(hipHostMalloc((void**)&rresp, sizeof(drw::ResResp) * drw::kResRing,
+                          hipHostMallocMapped | hipHostMallocCoherent));
+    DRW_HIP(hipHostMalloc((void**)&rstop, 2 * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(rreq, 0, sizeof(drw::ResReq) * drw::kResRing);
+    std::memset(rresp, 0, sizeof(drw::ResResp) * drw::kResRing);
+    rstop[0] = rstop[1] = 0;
+    rfree.reset(new std::atomic<uint64_t>[drw::kResRing]);
+    for (uint32_t i = 0; i < drw::kResRing; i++) rfree[i].store(i, std::memory_order_relaxed);
+    rowner.reset(new std::atomic<Req*>[drw::kResRing]);
+    for (uint32_t i = 0; i < drw::kResRing; i++) rowner[i].store(nullptr, std::memory_order_relaxed);
+    DRW_HIP(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
+    DRW_HIP(hipEventCreateWithFlags(&rdone, hipEventDisableTiming));
+    // fine-grained: the kernel's plain atomic loads of it are coherent across XCDs
+    DRW_HIP(hipExtMallocWithFlags((void**)&rctl, sizeof(drw::ResCtl), hipDeviceMallocFinegrained));
+    const drw::ResCtl zero{};
+    DRW_HIP(hipMemcpyAsync(rctl, &zero, sizeof zero, hipMemcpyHostToDevice, rstream));
+    DRW_HIP(hipStreamSynchronize(rstream));
+    DRW_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device));
+    if (rate_khz <= 0) return drw::set_error(DRW_EHIP, "server_create: no device wall clock");
+    rargs.ctl = rctl;
+    rargs.seed = cfg.seed;
+    // the dispatcher stops an idle kernel after resident_idle_ms; the kernel's
+    // own limit is only a safety net (a host that stopped polling)
+    const double idle_ms = cfg.resident_idle_ms > 0 ? cfg.resident_idle_ms : 20.0;
+    ridle = std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double, std::milli>(idle_ms));
+    rargs.idle_ticks = (uint64_t)(std::max(1000.0, 100.0 * idle_ms) * rate_khz);
+    rargs.scout_gap = (uint64_t)std::max(1, rate_khz / 1000);  // 1 us
+    DRW_HIP(hipHostGetDevicePointer((void**)&rargs.req, rreq, 0));
+    DRW_HIP(hipHostGetDevicePointer((void**)&rargs.resp, rresp, 0));
+    DRW_HIP(hipHostGetDevicePointer((void**)&rargs.host_stop, rstop, 0));
+    return 0;
+}
+
+void drw_server::res_destroy()
+{
+    if (rstop) __atomic_store_n(rstop, 1u, __ATOMIC_RELEASE);
+    if (rstream) (void)hipStreamSynchronize(rstream);
+    if (rdone) (void)hipEventDestroy(rdone);
+    if (rstream) (void)hipStreamDestroy(rstream);
+    if (rctl) (void)hipFree(rctl);
+    if (rnets) (void)hipFree(rnets);
+    if (rreq) (void)hipHostFree(rreq);
+    if (rresp) (void)hipHostFree(rresp);
+    if (rstop) (void)hipHostFree(rstop);
+    for (auto*& w : rw) {
+        drw::net_weights_free(w);
+        w = nullptr;
+    }
+}
+
+// (Re)starts the resident kernel: clears the stop flags and rewinds the
+// ticket head to the oldest unanswered sequence (tickets held by
+// workgroups of the last launch when it stopped are served again), then
+// launches.
+int drw_server::res_launch()
+{
+    __atomic_store_n(rstop, 0u, __ATOMIC_RELEASE);
+    const unsigned long long hs[2] = {rscan, 0};  // head, published
+    DRW_HIP(hipMemcpyAsync(&rctl->head, &hs[0], sizeof hs[0], hipMemcpyHostToDevice, rstream));
+    DRW_HIP(hipMemcpyAsync(&rctl->stop, &hs[1], sizeof hs[1], hipMemcpyHostToDevice, rstream));
+    DRW_HIP(hipStreamSynchronize(rstream));  // the copies have read hs
+    const int threads = cfg.resident_threads ? cfg.resident_threads : 512;
+    const int rc = drw::resident_launch(rargs, rnets, threads, rgrid, rstream);
+    if (rc) return rc;
+    DRW_HIP(hipEventRecord(rdone, rstream));
+    rrunning.store(true, std::memory_order_release);
+    rlaunches++;
+    return 0;
+}
+
+// Dispatcher step for the resident kernel: notice its end (then read where
+// its head stopped), stop it while FDM batches are pending or in flight,
+// and start it when published queries are waiting and no batch needs the
+// device.
+void drw_server::res_manage(int busy, int rout)
+{
+    const auto now = Clock::now();
+    if (rout > 0) rlast = now;
+    if (rrunning.load(std::memory_order_relaxed) && hipEventQuery(rdone) == hipSuccess)
+        rrunning.store(false, std::memory_order_release);
+    bool fdm = busy > 0, closing;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& qd : queue) fdm |= !qd.empty();
+        closing = stop;
+    }
+    if (rrunning.load(std::memory_order_relaxed)) {
+        // stop it for FDM batches, or when nothing has come for ridle (at
+        // once when the server is closing)
+        if (fdm || (rout == 0 && (closing || now - rlast > ridle))) __atomic_store_n(rstop, 1u, __ATOMIC_RELEASE);
+        return;
+    }
+    if (!fdm && rpub.load(std::memory_order_acquire) > rscan) {
+        rlast = now;
+        (void)res_launch();
+    }
+}
+
 void drw_server::loop()
 {
     (void)hipSetDevice(device);
@@ -228,6 +377,11 @@ void drw_server::loop()
             } else {
                 busy++;
             }
+        }
+        int rout = 0;  // resident queries published and not yet reaped
+        if (resident) {
+            rout = res_reap();
+            res_manage(busy, rout);
         }
         // 2) admissions: the class of the oldest pending request, limited by
         //    free slots, the batch cap and the HBM budget
@@ -265,16 +419,31 @@ void drw_server::loop()
             inflight_peak = std::max<uint64_t>(inflight_peak, (uint64_t)busy);
         }
         if (progressed) continue;
-        if (busy > 0) {
-            // batches in flight: keep polling their events (the dispatcher owns
-            // a core while work is on the device; completion latency is the
-            // serving latency)
+        if (busy > 0 || rout > 0) {
+            // batches or resident queries in flight: keep polling their
+            // events and the response ring (the dispatcher owns a core while
+            // work is on the device; completion latency is the serving
+            // latency)
             std::this_thread::yield();
             continue;
         }
         std::unique_lock<std::mutex> lk(mu);
         bool pending = false;
         for (auto& qd : queue) pending |= !qd.empty();
+        if (resident) {
+            // published resident queries not yet claimed when the kernel ended
+            const bool rwait = !rrunning.load(std::memory_order_relaxed) && rpub.load(std::memory_order_acquire) > rscan;
+            if (stop && !pending && !rwait && !rrunning.load(std::memory_order_relaxed)) return;
+            if (pending || rwait) continue;
+            // the kernel runs (or is idle-ended, seen at the next wake-up):
+            // check on it every millisecond
+            cv_work.wait_for(lk, std::chrono::milliseconds(1), [&] {
+                for (auto& qd : queue)
+                    if (!qd.empty()) return true;
+                return rpub.load(std::memory_order_acquire) > rscan;
+            });
+            continue;
+        }
         if (stop && !pending) return;
         cv_work.wait(lk, [&] {
             if (stop) return true;
@@ -324,16 +493,54 @@ int drw_server::submit(int n, const int32_t* kinds, const int32_t* Ns, const int
         q.t_submit = t_arrival ? t_arrival[i] : now;
         mine[i] = &q;
     }
+    auto release = [&mine] {
+        for (auto* q : mine) {
+            q->ticket.store(0, std::memory_order_relaxed);
+            q->state.store(kFree, std::memory_order_release);
+        }
+    };
+    int nres = 0;
+    for (auto* q : mine) nres += resident && q->cls % kRoutes == 0;
     {
         std::lock_guard<std::mutex> lk(mu);
         if (stop) {
-            for (auto* q : mine) {
-                q->ticket.store(0, std::memory_order_relaxed);
-                q->state.store(kFree, std::memory_order_release);
-            }
+            release();
             return drw::set_error(DRW_ESTATE, "server is shutting down");
         }
+    }
+    if (nres) {
+        // resident queries: sequences and publication, all or none (the
+        // kernel claims sequences in order, so none may stay unpublished);
+        // each request is kQueued before its seq word makes it visible
+        std::lock_guard<std::mutex> lr(rmu);
+        for (int j = 0; j < nres; j++)
+            if (rfree[(rnext + (uint64_t)j) & (drw::kResRing - 1)].load(std::memory_order_acquire) !=
+                rnext + (uint64_t)j) {
+                release();
+                return drw::set_error(DRW_ESTATE, "server_submit: resident ring full (wait for tickets)");
+            }
         for (auto* q : mine) {
+            if (q->cls % kRoutes != 0) continue;
+            const uint64_t sq = rnext++;
+            const uint32_t slot = (uint32_t)(sq & (drw::kResRing - 1));
+            q->rseq = sq;
+            q->res = true;
+            q->state.store(kQueued, std::memory_order_relaxed);
+            rowner[slot].store(q, std::memory_order_release);
+            drw::ResReq& r = rreq[slot];
+            std::memcpy(r.layers, q->in, sizeof(float) * 30);
+            std::memcpy(r.cuboids, q->in + 30, sizeof(float) * 105);
+            r.kind = q->kind;
+            r.axis = q->axis;
+            r.qid = q->qid;
+            __atomic_store_n(&r.seq, sq + 1, __ATOMIC_RELEASE);
+        }
+        rpub.store(rnext, std::memory_order_release);
+    }
+    if (nres < n) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto* q : mine) {
+            if (q->res) continue;
             q->state.store(kQueued, std::memory_order_relaxed);
             queue[q->cls].push_back(q);
         }
@@ -360,6 +567,74 @@ static int consume(Req* q, float* out)
     q->ticket.store(0, std::memory_order_relaxed);
     q->state.store(kFree, std::memory_order_release);
     return rc ? drw::set_error(rc, "server: batch failed") : 0;
+}
+
+// Dispatcher: collects the resident kernel's responses.  Sequences in
+// [rscan, published) are checked in order of age; a response whose six words
+// carry its tag completes its request as a batch completion does (result,
+// latency, kDone, waiters woken) and frees its ring slot; a NaN result
+// (status 1) is queued on the FDM route at N = 23 with the same counter,
+// which is what the batched network route's fallback computes for it.
+// Returns the number of requests still waiting for the kernel.
+int drw_server::res_reap()
+{
+    const uint64_t end = rpub.load(std::memory_order_acquire);
+    uint64_t first_open = end;
+    std::vector<Req*>& done = rdone_buf;
+    done.clear();
+    bool rerouted = false;
+    for (uint64_t sq = rscan; sq < end; sq++) {
+        const uint32_t slot = (uint32_t)(sq & (drw::kResRing - 1));
+        if (rfree[slot].load(std::memory_order_relaxed) != sq) continue;  // reaped already
+        const drw::ResResp& r = rresp[slot];
+        const uint32_t tag = (uint32_t)(sq + 1);
+        uint64_t w[drw::kResWords];
+        bool ready = true;
+        for (int j = 0; j < drw::kResWords && ready; j++) {
+            w[j] = __atomic_load_n(&r.w[j], __ATOMIC_ACQUIRE);
+            ready = (uint32_t)(w[j] >> 32) == tag;
+        }
+        if (!ready) {
+            first_open = std::min(first_open, sq);
+            continue;
+        }
+        Req* q = rowner[slot].load(std::memory_order_acquire);
+        for (int j = 0; j < 5; j++) {
+            const uint32_t v = (uint32_t)w[j];
+            std::memcpy(&q->out[j], &v, sizeof v);
+        }
+        q->res = false;
+        rfree[slot].store(sq + drw::kResRing, std::memory_order_release);
+        if ((uint32_t)w[5] != 0) {
+            std::lock_guard<std::mutex> lk(mu);
+            q->cls = q->kind * kRoutes + 3;
+            queue[q->cls].push_back(q);
+            rerouted = true;
+            continue;
+        }
+        q->rc = 0;
+        done.push_back(q);
+    }
+    rscan = first_open;
+    if (!done.empty()) {
+        const auto now = Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(mu_stats);
+            for (Req* q : done)
+                if (lat_ms.size() < (1u << 22))
+                    lat_ms.push_back(std::chrono::duration<float, std::milli>(now - q->t_submit).count());
+            completed += done.size();
+        }
+        {
+            // the waiter frees an entry once it sees kDone: nothing here
+            // touches a request after its state store
+            std::lock_guard<std::mutex> lk(mu_done);
+            for (Req* q : done) q->state.store(kDone, std::memory_order_release);
+        }
+        cv_done.notify_all();
+    }
+    (void)rerouted;
+    return (int)std::min<uint64_t>(end - rscan, 1u << 30);
 }
 
 int drw_server::take(uint64_t ticket, float* out, int timeout_ms)
@@ -400,8 +675,12 @@ int drw_server_create(int device, const char* models_txt, const drw_server_confi
     c.seed = 0x5eed5eedULL;
     if (cfg) c = *cfg;
     if (c.max_batch < 1 || c.max_batch > 65536 || c.max_inflight < 1 || c.max_inflight > 16 || !(c.hbm_budget_mb > 0) ||
-        !(c.fdm_tol > 0))
+        !(c.fdm_tol > 0) || (c.resident != 0 && c.resident != 1) ||
+        (c.resident_threads != 0 && c.resident_threads != 256 && c.resident_threads != 512) ||
+        !(c.resident_idle_ms >= 0))
         return drw::set_error(DRW_EINVAL, "server_create: bad configuration");
+    if (c.resident && !(models_txt && models_txt[0]))
+        return drw::set_error(DRW_EINVAL, "server_create: the resident loop serves the networks: give a model list");
     int ndev = 0;
     DRW_HIP(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return drw::set_error(DRW_EINVAL, "server_create: bad device %d", device);
@@ -446,6 +725,15 @@ int drw_server_create(int device, const char* models_txt, const drw_server_confi
                     return drw::set_error(rc, "%s", msg.c_str());
                 }
             }
+    }
+    if (c.resident) {
+        sv->resident = true;
+        const int rc = sv->res_create();
+        if (rc) {
+            const std::string msg = drw::last_error();
+            drw_server_destroy(sv);
+            return drw::set_error(rc, "%s", msg.c_str());
+        }
     }
     sv->t0 = Clock::now();
     sv->worker = std::thread([sv] { sv->loop(); });
@@ -497,6 +785,26 @@ int drw_server_wait_many(drw_server* sv, int n, const uint64_t* tickets, float* 
         if (rc && !first) first = rc;
     }
     return first ? drw::set_error(first, "server_wait_many: %s", drw::last_error()) : 0;
+}
+
+int drw_server_resident_stats(drw_server* sv, uint64_t* out)
+{
+    drw::clear_error();
+    if (!sv || !out) return drw::set_error(DRW_EINVAL, "server_resident_stats: null argument");
+    std::memset(out, 0, 10 * sizeof(uint64_t));
+    if (!sv->resident) return 0;
+    (void)hipSetDevice(sv->device);
+    drw::ResCtl c{};
+    hipStream_t st = nullptr;
+    DRW_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipError_t e = hipMemcpyAsync(&c, sv->rctl, sizeof c, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (e != hipSuccess) return drw::set_error(DRW_EHIP, "server_resident_stats: %s", hipGetErrorString(e));
+    const uint64_t v[10] = {sv->rlaunches.load(), c.served[0], c.served[1], c.scouts, c.exits_idle, c.exits_stop,
+                            c.head, c.published, (uint64_t)sv->rgrid, (uint64_t)sv->rate_khz};
+    std::memcpy(out, v, sizeof v);
+    return 0;
 }
 
 int drw_server_stats_get(drw_server* sv, drw_server_stats* st, int reset)
@@ -638,6 +946,7 @@ void drw_server_destroy(drw_server* sv)
     sv->cv_work.notify_all();
     if (sv->worker.joinable()) sv->worker.join();  // drains the queues first
     (void)hipSetDevice(sv->device);
+    sv->res_destroy();
     for (auto& s : sv->slots) {
         if (s.stream) (void)hipStreamSynchronize(s.stream);
         for (auto* h : s.solvers)

@@ -11,7 +11,8 @@ from ._lib import check
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_int), ("max_inflight", ctypes.c_int), ("hbm_budget_mb", ctypes.c_double),
-                ("fdm_tol", ctypes.c_double), ("seed", ctypes.c_uint64)]
+                ("fdm_tol", ctypes.c_double), ("seed", ctypes.c_uint64), ("resident", ctypes.c_int),
+                ("resident_threads", ctypes.c_int), ("resident_idle_ms", ctypes.c_double)]
 
 
 class ServerStats(ctypes.Structure):
@@ -42,8 +43,11 @@ class Server:
     """Thread-safe continuous-batching front end for transition queries."""
 
     def __init__(self, device=0, models_txt=None, max_batch=2048, max_inflight=4, hbm_budget_mb=1024.0,
-                 fdm_tol=1e-6, seed=0x5EED5EED):
-        cfg = ServerConfig(max_batch, max_inflight, hbm_budget_mb, fdm_tol, seed)
+                 fdm_tol=1e-6, seed=0x5EED5EED, resident=False, resident_threads=512, resident_idle_ms=20.0):
+        """resident: N = 23 queries go to the resident loop (a persistent
+        kernel reading a pinned request ring) instead of batches."""
+        cfg = ServerConfig(max_batch, max_inflight, hbm_budget_mb, fdm_tol, seed, int(bool(resident)),
+                           resident_threads, resident_idle_ms)
         h = ctypes.c_void_p()
         check(_lib.lib().drw_server_create(device, models_txt.encode() if models_txt else None, ctypes.byref(cfg),
                                            ctypes.byref(h)), "drw_server_create")
@@ -105,6 +109,14 @@ class Server:
         s = ServerStats()
         check(_lib.lib().drw_server_stats_get(self._h, ctypes.byref(s), int(reset)), "drw_server_stats_get")
         return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def resident_stats(self) -> dict:
+        """Resident-loop counters (drw_server_resident_stats)."""
+        out = (ctypes.c_uint64 * 10)()
+        check(_lib.lib().drw_server_resident_stats(self._h, out), "drw_server_resident_stats")
+        keys = ("launches", "served_poisson", "served_gradient", "scouts", "exits_idle", "exits_stop", "claimed",
+                "published", "grid", "wall_clock_khz")
+        return dict(zip(keys, (int(v) for v in out)))
 
     def run_load(self, layers, cuboids, kinds, Ns, axes, threads=64, rate_qps=0.0, duration_s=2.0,
                  max_outstanding=64, seed=1) -> dict:

@@ -227,6 +227,17 @@ typedef struct {
     double hbm_budget_mb;  /* device memory for in-flight queries (default 1024) */
     double fdm_tol;        /* FDM relative residual (default 1e-6) */
     uint64_t seed;         /* Philox key */
+    /* Resident loop (no counterpart in the reference): with resident = 1 and
+     * a model list, N = 23 queries of both kinds skip the batches; a
+     * persistent kernel takes them from a pinned request ring as they are
+     * submitted, serves each in one workgroup and writes the result back,
+     * and waiters read it from the ring.  FDM-route queries (N = 16, 21)
+     * still run as batches: the dispatcher stops the resident kernel while
+     * they run and restarts it afterwards.  Results are bit-identical to the
+     * batched route's for the same counter. */
+    int resident;              /* 0: batches only (default); 1: resident loop for N = 23 */
+    int resident_threads;      /* threads per resident workgroup, 256 or 512 (0: 512) */
+    double resident_idle_ms;   /* the resident kernel ends after this long without queries (0: 20 ms) */
 } drw_server_config;
 typedef struct {
     uint64_t completed, batches, inflight_peak;
@@ -254,6 +265,12 @@ int drw_server_submit_many(drw_server* sv, int n, const int32_t* kinds, const in
                            const float* layers, const float* cuboids, const uint64_t* qids, uint64_t* tickets);
 int drw_server_wait_many(drw_server* sv, int n, const uint64_t* tickets, float* outs, int timeout_ms);
 int drw_server_stats_get(drw_server* sv, drw_server_stats* stats, int reset);
+/* Resident-loop counters (zeros for a batches-only server), out[10]: kernel
+ * launches, Poisson and gradient queries served, scout polls of the request
+ * ring, workgroups that left idle and on the stop flag, claimed and
+ * published sequences, workgroups per launch, device wall clock (kHz).  No
+ * counterpart in the reference. */
+int drw_server_resident_stats(drw_server* sv, uint64_t* out);
 /* Drains pending queries, then frees everything. */
 void drw_server_destroy(drw_server* sv);
 
