@@ -521,7 +521,8 @@ def main():
         rates = [1e5, 1e6, 2e6, 3e6, 5e6, 1e7, 0]
         res = {"serving_device_ceiling": device_mix_ceiling(models_txt),
                "serving": serving_runs(models_txt, rates, duration_s=1.5),
-               "serving_resident": serving_runs(models_txt, rates, duration_s=1.5, resident=True)}
+               "serving_resident": serving_runs(models_txt, rates, duration_s=1.5, resident=True),
+               "serving_mixed_n_resident": serving_runs(models_txt, [1e5, 1e6], mixed_n=True, resident=True)}
         print(json.dumps(res), flush=True)
         return
     B = args.batch
@@ -644,6 +645,7 @@ def main():
         res["max_offered_qps_with_p99_under_1ms"] = max(sustained) if sustained else None
         res["serving_mixed_n"] = serving_runs(models_txt, [1e5, 1e6], mixed_n=True)
         res["serving_resident"] = serving_runs(models_txt, rates, duration_s=1.5, resident=True)
+        res["serving_mixed_n_resident"] = serving_runs(models_txt, [1e5, 1e6], mixed_n=True, resident=True)
         res["p99_latency_ms_resident"] = {
             ("back_to_back" if r["offered_qps"] is None else f"offered_{r['offered_qps']:.0e}"): r["p99_ms"]
             for r in res["serving_resident"]}

@@ -57,6 +57,7 @@ namespace {
 constexpr int kIn = drw::kMaxLayers * 2 + drw::kMaxCuboids * 7;  // 135 floats
 constexpr int kRoutes = 4;                                        // NN23, FDM16, FDM21, FDM23
 constexpr uint64_t kRing = 1u << 16;  // tickets outstanding (submitted, not yet waited for) at most
+constexpr int64_t kFdmHoldNs = 5000000;  // resident mode: network queries batch for 5 ms after an FDM-route query
 
 enum : int { kFree = 0, kClaimed = 1, kQueued = 2, kDone = 3 };
 
@@ -151,6 +152,7 @@ struct drw_server {
     std::vector<Req*> rdone_buf;          // dispatcher: requests completed in one reaper pass
     std::atomic<bool> rrunning{false};    // resident kernel launched and not seen to end (dispatcher-owned)
     std::atomic<uint64_t> rlaunches{0};
+    std::atomic<int64_t> rfdm_last{INT64_MIN / 2};  // steady-clock ns of the last FDM-route submit
     int rate_khz = 0;                     // device wall clock
     Clock::duration ridle{};              // the dispatcher stops the kernel after this long idle
     Clock::time_point rlast{};            // last resident submit or completion
@@ -158,7 +160,7 @@ struct drw_server {
     int slot_solver(Slot& s, int cls, drw_solver** out);
     int res_create();
     void res_destroy();
-    int res_launch();
+    int res_launch(bool drain);
     void res_manage(int busy, int rout);
     int res_reap();
     int launch(Slot& s, int cls, std::vector<Req*>& batch);
@@ -313,10 +315,12 @@ void drw_server::res_destroy()
 // ticket head to the oldest unanswered sequence (tickets held by
 // workgroups of the last launch when it stopped are served again), then
 // launches.
-int drw_server::res_launch()
+int drw_server::res_launch(bool drain)
 {
-    __atomic_store_n(rstop, 0u, __ATOMIC_RELEASE);
-    const unsigned long long hs[2] = {rscan, 0};  // head, published
+    // a drain launch starts with the stop flag set: its workgroups serve
+    // what is published and leave
+    __atomic_store_n(rstop, drain ? 1u : 0u, __ATOMIC_RELEASE);
+    const unsigned long long hs[2] = {rscan, drain ? 1ull : 0ull};  // head, stop
     DRW_HIP(hipMemcpyAsync(&rctl->head, &hs[0], sizeof hs[0], hipMemcpyHostToDevice, rstream));
     DRW_HIP(hipMemcpyAsync(&rctl->stop, &hs[1], sizeof hs[1], hipMemcpyHostToDevice, rstream));
     DRW_HIP(hipStreamSynchronize(rstream));  // the copies have read hs
@@ -329,10 +333,11 @@ int drw_server::res_launch()
     return 0;
 }
 
-// Dispatcher step for the resident kernel: notice its end (then read where
-// its head stopped), stop it while FDM batches are pending or in flight,
-// and start it when published queries are waiting and no batch needs the
-// device.
+// Dispatcher step for the resident kernel: notice its end, stop it while
+// batches are pending or in flight (its workgroups hold every CU), and
+// start it when published queries are waiting: as a resident loop when no
+// batch needs the device, else as a drain launch that serves what is
+// published and ends.
 void drw_server::res_manage(int busy, int rout)
 {
     const auto now = Clock::now();
@@ -351,9 +356,9 @@ void drw_server::res_manage(int busy, int rout)
         if (fdm || (rout == 0 && (closing || now - rlast > ridle))) __atomic_store_n(rstop, 1u, __ATOMIC_RELEASE);
         return;
     }
-    if (!fdm && rpub.load(std::memory_order_acquire) > rscan) {
+    if (rpub.load(std::memory_order_acquire) > rscan) {
         rlast = now;
-        (void)res_launch();
+        (void)res_launch(fdm);
     }
 }
 
@@ -499,8 +504,20 @@ int drw_server::submit(int n, const int32_t* kinds, const int32_t* Ns, const int
             q->state.store(kFree, std::memory_order_release);
         }
     };
+    // While FDM-route queries are about (submitted within the last
+    // kFdmHold), network queries take the batches too: the resident kernel
+    // holds every CU, so it is stopped while batches run.  Results are the
+    // same either way (per-query counters).
+    bool batch_nn = false;
+    if (resident) {
+        const int64_t t_now = std::chrono::duration_cast<std::chrono::nanoseconds>(now.time_since_epoch()).count();
+        bool has_fdm = false;
+        for (auto* q : mine) has_fdm |= q->cls % kRoutes != 0;
+        if (has_fdm) rfdm_last.store(t_now, std::memory_order_relaxed);
+        batch_nn = has_fdm || t_now - rfdm_last.load(std::memory_order_relaxed) < kFdmHoldNs;
+    }
     int nres = 0;
-    for (auto* q : mine) nres += resident && q->cls % kRoutes == 0;
+    for (auto* q : mine) nres += resident && !batch_nn && q->cls % kRoutes == 0;
     {
         std::lock_guard<std::mutex> lk(mu);
         if (stop) {
@@ -520,7 +537,7 @@ int drw_server::submit(int n, const int32_t* kinds, const int32_t* Ns, const int
                 return drw::set_error(DRW_ESTATE, "server_submit: resident ring full (wait for tickets)");
             }
         for (auto* q : mine) {
-            if (q->cls % kRoutes != 0) continue;
+            if (q->cls % kRoutes != 0) continue;  // nres > 0: every network query goes to the ring
             const uint64_t sq = rnext++;
             const uint32_t slot = (uint32_t)(sq & (drw::kResRing - 1));
             q->rseq = sq;
