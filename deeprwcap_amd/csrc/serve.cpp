@@ -133,6 +133,7 @@ struct drw_server {
     std::thread worker;
     // resident loop (cfg.resident)
     bool resident = false;
+    std::atomic<bool> resident_ok{true};  // false after a failed launch: network queries take the batches
     drw::NetWeights* rw[2] = {nullptr, nullptr};
     void* rnets = nullptr;          // device views of both model sets
     drw::ResReq* rreq = nullptr;    // pinned request ring (host view)
@@ -161,6 +162,7 @@ struct drw_server {
     int res_create();
     void res_destroy();
     int res_launch(bool drain);
+    void res_abandon();
     void res_manage(int busy, int rout);
     int res_reap();
     int launch(Slot& s, int cls, std::vector<Req*>& batch);
@@ -333,6 +335,25 @@ int drw_server::res_launch(bool drain)
     return 0;
 }
 
+// After a failed launch (no kernel runs): the ring's unanswered queries go
+// to the batched network route (same results: the counters are per query),
+// and so do later ones.
+void drw_server::res_abandon()
+{
+    resident_ok.store(false, std::memory_order_release);
+    const uint64_t end = rpub.load(std::memory_order_acquire);
+    std::lock_guard<std::mutex> lk(mu);
+    for (uint64_t sq = rscan; sq < end; sq++) {
+        const uint32_t slot = (uint32_t)(sq & (drw::kResRing - 1));
+        if (rfree[slot].load(std::memory_order_relaxed) != sq) continue;
+        Req* q = rowner[slot].load(std::memory_order_acquire);
+        rfree[slot].store(sq + drw::kResRing, std::memory_order_release);
+        q->res = false;
+        queue[q->cls].push_back(q);
+    }
+    rscan = end;
+}
+
 // Dispatcher step for the resident kernel: notice its end, stop it while
 // batches are pending or in flight (its workgroups hold every CU), and
 // start it when published queries are waiting: as a resident loop when no
@@ -358,7 +379,7 @@ void drw_server::res_manage(int busy, int rout)
     }
     if (rpub.load(std::memory_order_acquire) > rscan) {
         rlast = now;
-        (void)res_launch(fdm);
+        if (res_launch(fdm)) res_abandon();
     }
 }
 
@@ -508,8 +529,8 @@ int drw_server::submit(int n, const int32_t* kinds, const int32_t* Ns, const int
     // kFdmHold), network queries take the batches too: the resident kernel
     // holds every CU, so it is stopped while batches run.  Results are the
     // same either way (per-query counters).
-    bool batch_nn = false;
-    if (resident) {
+    bool batch_nn = resident && !resident_ok.load(std::memory_order_acquire);
+    if (resident && !batch_nn) {
         const int64_t t_now = std::chrono::duration_cast<std::chrono::nanoseconds>(now.time_since_epoch()).count();
         bool has_fdm = false;
         for (auto* q : mine) has_fdm |= q->cls % kRoutes != 0;
