@@ -6,11 +6,14 @@ events on the launch stream):
 * launch shapes: threads per workgroup of each face-predictor kernel (256 or
   512 per (query, face); a wider workgroup shares one query's LDS among more
   waves, which pays where the kernel is latency-bound: small batches, and the
-  gradient kernels, whose LDS allows only two workgroups per CU).  Every
-  kernel is timed at both shapes over a ladder of batch sizes and the winners
-  become the library's per-kernel rules (drw_set_launch_shape: from
-  `min_items` workgroups on, use `threads`); a batch size switches shape only
-  when the other one is faster by more than 2 %;
+  gradient kernels, whose LDS allows only two workgroups per CU), and for
+  Poisson queries the fused query kernel at 256 or 512 threads against the
+  separate selector, head and predictor kernels (shape 1).  Every kernel is
+  timed at each shape over a ladder of batch sizes (the Poisson query as a
+  whole: the sum of its stages) and the winners become the library's
+  per-kernel rules (drw_set_launch_shape: from `min_items` workgroups on,
+  use `threads`); a batch size switches shape only when another one is
+  faster by more than 2 %;
 * batch shape: with those shapes, one batch of every route the server uses
   (network Poisson / gradient queries at N = 23, FDM Poisson / gradient
   queries at N = 16, 21, 23) over a ladder of batch sizes, and the server's
@@ -41,7 +44,8 @@ LADDER = (64, 128, 256, 512, 1024, 2048, 4096)
 SHAPE_LADDER = (64, 128, 256, 512, 768, 1024, 1536, 2048, 4096, 8192)
 SHAPES = (256, 512)
 # drw_shape ids (include/dnnsolver_c.h)
-SHAPE_KERNELS = {"face_poisson": 0, "face_grad1": 1, "face_grad2": 2}
+SHAPE_KERNELS = {"face_poisson": 0, "face_grad1": 1, "face_grad2": 2, "poisson_fused": 3}
+FUSED_SHAPES = (1, 256, 512)  # 1: the separate kernels
 SWITCH_MARGIN = 0.02
 
 
@@ -80,7 +84,9 @@ def rules_from_timings(times, margin=SWITCH_MARGIN):
     return rules
 
 
-def _stage_ms(solver, run, stage, reps, torch):
+def _stage_ms(solver, run, stages, reps, torch):
+    """Mean milliseconds per launch of one stage, or the sum over a tuple of
+    stages (per launch of the first that ran)."""
     import ctypes
     lib = _lib.lib()
     ms = (ctypes.c_double * 8)()
@@ -93,7 +99,9 @@ def _stage_ms(solver, run, stage, reps, torch):
         run()
     torch.cuda.synchronize()
     _lib.check(lib.drw_solver_stage_times(solver.handle, ms, cnt, 1), "stage_times")
-    return ms[stage] / max(cnt[stage], 1)
+    stages = (stages,) if isinstance(stages, int) else stages
+    n = max([cnt[k] for k in stages] + [1])
+    return sum(ms[k] for k in stages) / n
 
 
 def tune_shapes(models_txt, ladder=SHAPE_LADDER, device=0, log=print):
@@ -119,11 +127,20 @@ def tune_shapes(models_txt, ladder=SHAPE_LADDER, device=0, log=print):
                            "sample_device")
             reps = max(10, min(60, int(4e5 / B)))
             if kind == 0:
+                # separate kernels (the face kernel at each shape), then the
+                # fused query kernel at each shape; whole-query times
+                set_shape_rules(3, [(0, 1)])
                 t = {}
                 for th in SHAPES:
                     set_shape_rules(0, [(0, th)])
                     t[th] = _stage_ms(s, run, 2, reps, torch)
                 times["face_poisson"][B] = t
+                set_shape_rules(0, [(0, min(t, key=t.get))])
+                tf = {1: _stage_ms(s, run, (0, 1, 2), reps, torch)}
+                for th in SHAPES:
+                    set_shape_rules(3, [(0, th)])
+                    tf[th] = _stage_ms(s, run, 7, reps, torch)
+                times["poisson_fused"][B] = tf
             else:
                 # the stage runs Gradient1 then Gradient2: vary one kernel at a time
                 base = {}
@@ -134,8 +151,9 @@ def tune_shapes(models_txt, ladder=SHAPE_LADDER, device=0, log=print):
                 t0 = base[(256, 256)]
                 times["face_grad1"][B] = {256: t0, 512: base[(512, 256)]}
                 times["face_grad2"][B] = {256: t0, 512: base[(256, 512)]}
-            log(f"shape B={B}: " + ", ".join(f"{k} " + "/".join(f"{v[B][th]:.4f}" for th in SHAPES)
-                                               for k, v in times.items() if B in v) + " ms (256/512 threads)")
+            log(f"shape B={B}: " + ", ".join(f"{k} " + "/".join(f"{v[B][th]:.4f}" for th in sorted(v[B]))
+                                               for k, v in times.items() if B in v)
+                + " ms (256/512 threads; poisson_fused: separate/256/512)")
         del s
     rules = {k: rules_from_timings(v) for k, v in times.items()}
     for k, r in rules.items():

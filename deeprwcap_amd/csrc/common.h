@@ -22,7 +22,15 @@ int set_error(int code, const char* fmt, ...);
 // threads per workgroup of the tunable kernels as rules (min_items,
 // threads); a launch of `items` workgroups takes the rule with the largest
 // min_items <= items, else 256 threads.
-enum LaunchShape : int { kShapeFacePoisson = 0, kShapeFaceGrad1 = 1, kShapeFaceGrad2 = 2, kShapeCount = 3 };
+// kShapePoissonFused: the fused Poisson query kernel's threads (256, 512),
+// or 1 for the separate selector, head and predictor kernels.
+enum LaunchShape : int {
+    kShapeFacePoisson = 0,
+    kShapeFaceGrad1 = 1,
+    kShapeFaceGrad2 = 2,
+    kShapePoissonFused = 3,
+    kShapeCount = 4
+};
 int launch_shape(int which, int items);
 int set_launch_shape(int which, int min_items, int threads);
 

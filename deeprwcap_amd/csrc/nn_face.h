@@ -42,9 +42,18 @@ struct FaceQueryArgs {
     int32_t* fb_count;
     int fb_cap;
     uint64_t* fb_stats;
+    float* probs_out;        // fused Poisson query: [B, 6] face probabilities (optional)
+    int32_t* face_out;       // fused Poisson query: [B] drawn face (optional)
 };
 
 // mode 0: PoissonPredictor, 1: Gradient1Predictor, 2: Gradient2Predictor.
 int face_query_launch(int mode, const FaceQueryArgs& a, const FaceNetDev& w, hipStream_t stream);
+
+struct SelNetDev;
+// The whole Poisson query in one kernel (sample mode): one workgroup per
+// query runs the PoissonSelector, its Linear head, the face draw and the
+// PoissonPredictor on that face, with threads = 256 or 512 per workgroup.
+int poisson_query_launch(const FaceQueryArgs& a, const SelNetDev& sel, const FaceNetDev& pred, int threads,
+                         hipStream_t stream);
 
 }  // namespace drw

@@ -14,10 +14,13 @@
 // else 16x16x16; fp32 accumulate); depthwise 3x3, the small 1x1 layers,
 // batch-norm (folded into weights/bias on the host) and GELU (LDS table) run
 // on the VALU in fp32.
+#include <cstddef>
+
 #include "common.h"
 #include "nn_common.h"
 #include "nn_face.h"
 #include "nn_face_body.h"
+#include "nn_sel_body.h"
 #include "sampling.h"
 
 namespace drw {
@@ -88,6 +91,104 @@ __global__ __launch_bounds__(NTH) void face_query_kernel(FaceQueryArgs a, FaceNe
             if (nan) fallback_append(a, b);
         }
     }
+}
+
+// ------------------------------------------------------- fused Poisson query
+// LDS of the fused kernel: the selector phase (im2col offsets / GELU table,
+// cube, features) and the head's activations overlay the predictor's
+// activation buffers X, Y; the predictor's description, GELU table and zero
+// slot sit beyond them and are filled once, at the start.  39.4 KB: four
+// 256-thread workgroups per CU, as for the predictor kernel alone.
+union PoissonSmem {
+    FaceSmem<16> f;
+    struct {
+        alignas(16) int koff[27 * 16];
+        half_t cube[SelSmem<4, 8>::kRegion];
+        alignas(16) float feat[128];
+    } sel;
+    struct {
+        alignas(16) half_t x[128];
+        alignas(16) half_t y1[64];
+        alignas(16) float logits[16];
+    } head;
+};
+static_assert(sizeof(PoissonSmem::sel) <= offsetof(FaceSmem<16>, zero), "selector phase must not reach the predictor's tables");
+static_assert(sizeof(PoissonSmem) == sizeof(FaceSmem<16>), "the fused kernel's LDS is the predictor's");
+
+// One workgroup per query: description; PoissonSelector backbone and Linear
+// head (the batched kernels' device code, so every value is the separate
+// kernels'); softmax and face draw (word 0); PoissonPredictor on the face;
+// output head, tile draw (word 1), locate (dnn_solver_poisson.cpp:86-111).
+template <int NTH>
+__global__ __launch_bounds__(NTH) void poisson_query_kernel(FaceQueryArgs a, SelNetDev sw, FaceNetDev pw)
+{
+    __shared__ PoissonSmem s;
+    FaceSmem<16>& f = s.f;
+    const int b = blockIdx.x;
+    if (threadIdx.x < 64) vox_desc_build(f.desc, a.layers, a.cuboids, b);
+    else
+        for (int i = threadIdx.x - 64; i < kSelGeluTab; i += NTH - 64) ((float2*)s.sel.koff)[i] = sw.gtab[i];
+    for (int i = threadIdx.x; i < kGeluTabEntries; i += NTH) f.gtab[i] = pw.gtab[i];
+    if (threadIdx.x < 4) f.zero[threadIdx.x] = 0.f;
+    __syncthreads();
+    selector_backbone_body<false, NTH>(f.desc, s.sel.cube, s.sel.koff, false, 2, sw, s.sel.feat);
+    for (int i = threadIdx.x; i < 128; i += NTH) s.head.x[i] = (half_t)s.sel.feat[i];
+    __syncthreads();
+    linear_row<128, 64, true, NTH>(s.head.x, s.head.y1, nullptr, sw.frag[3], sw.bias[3]);
+    __syncthreads();
+    linear_row<64, 6, false, NTH>(s.head.y1, nullptr, s.head.logits, sw.frag[4], sw.bias[4]);
+    __syncthreads();
+    const uint64_t qid = a.qid ? a.qid[b] : (a.ctr_dev ? *a.ctr_dev : a.counter) + (uint64_t)b;
+    if (threadIdx.x == 0) {
+        float p[6];
+        const int face = selector_probs_draw(s.head.logits, a.seed, qid, p);
+        f.ibuf[0] = face;
+        if (a.probs_out)
+            for (int i = 0; i < 6; i++) a.probs_out[(size_t)b * 6 + i] = p[i];
+        if (a.face_out) a.face_out[b] = face;
+    }
+    __syncthreads();
+    const int face = f.ibuf[0];
+    if (face < 0) {  // no drawable face: a NaN result, for the FDM fallback
+        if (threadIdx.x < 3) a.out[(size_t)b * 3 + threadIdx.x] = __builtin_nanf("");
+        if (threadIdx.x == 0) fallback_append(a, b);
+        return;
+    }
+    face_solver<NTH, 0>(f, pw, face, 2);
+    __syncthreads();
+    float* kern = face_output_kernel<NTH, 0>(f, face, 2);
+    if (a.kern_out) {
+        float* o = a.kern_out + (size_t)b * kNN;
+        for (int p = threadIdx.x; p < kNN; p += NTH) o[p] = kern[p];
+    }
+    if (threadIdx.x < 64) {
+        uint32_t u[4];
+        query_uniforms(a.seed, qid, 0u, u);
+        const int t = wave_cdf_draw<float>(kern, kNN, u[1]);
+        if (threadIdx.x == 0) {
+            float r[5];
+            face_result<0>(kern, f.desc, t, face, 2, 0.f, r);
+            bool nan = false;
+            for (int j = 0; j < 3; j++) {
+                a.out[(size_t)b * 3 + j] = r[j];
+                nan |= isnan(r[j]);
+            }
+            if (nan) fallback_append(a, b);
+        }
+    }
+}
+
+int poisson_query_launch(const FaceQueryArgs& a, const SelNetDev& sel, const FaceNetDev& pred, int threads,
+                         hipStream_t stream)
+{
+    if (a.B <= 0) return 0;
+    if (a.full) return set_error(-2, "poisson_query_launch: sample mode only");
+    if (threads == 512)
+        hipLaunchKernelGGL(poisson_query_kernel<512>, dim3(a.B), dim3(512), 0, stream, a, sel, pred);
+    else
+        hipLaunchKernelGGL(poisson_query_kernel<256>, dim3(a.B), dim3(256), 0, stream, a, sel, pred);
+    DRW_HIP(hipGetLastError());
+    return 0;
 }
 
 int face_query_launch(int mode, const FaceQueryArgs& a, const FaceNetDev& w, hipStream_t stream)

@@ -24,7 +24,8 @@ int get_workspace(hipStream_t stream, Workspace** out);
 // Optional per-stage kernel timing with HIP events recorded on the launch
 // stream around each stage (no synchronisation until collect()).
 // Stages: 0 selector backbone, 1 selector head, 2 face predictor,
-// 3 voxelize, 4 FDM solve, 5 flat sampler.
+// 3 voxelize, 4 FDM solve, 5 flat sampler, 6 network fallback, 7 fused
+// Poisson query.
 struct StageTimer {
     static constexpr int kStages = 8;
     bool enabled = false;

@@ -123,7 +123,7 @@ int drw_solver_output_width(const drw_solver* s);
  * counterpart of the walker's "GPU Sampler Statistics",
  * executable/run_script.py:108-162).  Stages: 0 selector backbone,
  * 1 selector head, 2 face predictor, 3 voxelize, 4 FDM solve, 5 flat
- * sampler.  stage_times waits for the recorded events and returns the
+ * sampler, 6 network fallback, 7 fused Poisson query.  stage_times waits for the recorded events and returns the
  * accumulated milliseconds and launch counts of 8 stages (reset != 0 clears). */
 int drw_solver_set_timing(drw_solver* s, int enable);
 int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int reset);
@@ -139,8 +139,17 @@ int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int
  * autotuner's choice on MI355X).  get returns the threads a launch of
  * `items` workgroups uses.  Chosen per machine by the autotuner
  * (deeprwcap_amd.autotune).  DRW_SHAPE_FACE_* are the face predictors'
- * kernels (one workgroup per query and face, 256 or 512 threads). */
-enum drw_shape { DRW_SHAPE_FACE_POISSON = 0, DRW_SHAPE_FACE_GRAD1 = 1, DRW_SHAPE_FACE_GRAD2 = 2 };
+ * kernels (one workgroup per query and face, 256 or 512 threads).
+ * DRW_SHAPE_POISSON_FUSED is the Poisson query in one kernel (sample mode:
+ * selector, head, face draw and predictor in one workgroup per query, 256 or
+ * 512 threads), or 1: the separate selector, head and predictor kernels;
+ * both give bit-identical results. */
+enum drw_shape {
+    DRW_SHAPE_FACE_POISSON = 0,
+    DRW_SHAPE_FACE_GRAD1 = 1,
+    DRW_SHAPE_FACE_GRAD2 = 2,
+    DRW_SHAPE_POISSON_FUSED = 3
+};
 int drw_set_launch_shape(int which, int min_items, int threads);
 int drw_get_launch_shape(int which, int items);
 

@@ -10,7 +10,7 @@ def lib():
     from deeprwcap_amd import _lib
     L = _lib.lib()
     yield L
-    for w in range(3):
+    for w in range(4):
         assert L.drw_set_launch_shape(w, -1, 0) == 0  # back to the built-in rules
 
 
@@ -21,6 +21,10 @@ def test_builtin_rules(lib):
         [512, 512, 512, 256, 512, 256, 256]
     for w in (1, 2):
         assert [lib.drw_get_launch_shape(w, b) for b in (1, 2048, 65536)] == [512] * 3
+    # the fused Poisson query kernel below 4096 queries, the separate kernels
+    # (shape 1) from there on
+    assert [lib.drw_get_launch_shape(3, b) for b in (1, 1023, 1024, 1536, 4095, 4096, 8192)] == \
+        [512, 512, 256, 512, 512, 1, 1]
 
 
 def test_rule_semantics(lib):
@@ -41,7 +45,9 @@ def test_rule_semantics(lib):
 def test_rule_errors(lib):
     assert lib.drw_set_launch_shape(0, 0, 300) == -2
     assert b"256 or 512" in lib.drw_last_error()
-    assert lib.drw_set_launch_shape(3, 0, 256) == -2
+    assert lib.drw_set_launch_shape(4, 0, 256) == -2           # no such kernel
+    assert lib.drw_set_launch_shape(0, 0, 1) == -2             # 1 (separate kernels) is the fused kernel's only
+    assert lib.drw_set_launch_shape(3, 0, 1) == 0
     assert lib.drw_get_launch_shape(-1, 5) == -2
     assert lib.drw_set_launch_shape(1, -1, 256) == 0           # one rule: (0, 256)
     for k in range(7):
