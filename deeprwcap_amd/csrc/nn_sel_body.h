@@ -50,6 +50,46 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 #pragma unroll
             for (int kt = 0; kt < KT; kt++) bw[n][kt] = frag[(n * KT + kt) * 64 + lane];
     }
+    if constexpr (NT > 1 && TP < NW) {
+        // fewer row tiles than waves (FaceSelectorWeight's last layer: one
+        // tile of 8 positions, 64 channels): one (row tile, channel tile)
+        // per wave, so every wave works; each product is the same MFMA chain
+        // over k as below
+        constexpr int ITEMS = TP * NT, IPW = (ITEMS + NW - 1) / NW;
+#pragma unroll
+        for (int u = 0; u < IPW; u++) {
+            const int item = wave + NW * u;
+            if (item >= ITEMS) continue;
+            const int t = item / NT, n = item - t * NT;
+            const int m = t * 16 + r;
+            int base = -1;
+            if (m < M) {
+                const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
+                base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
+            }
+            float4_t acc = {0.f, 0.f, 0.f, 0.f};
+            for (int kt = 0; kt < KT; kt++) {
+                const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
+                const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
+                half4_t a;
+#pragma unroll
+                for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
+                acc = mfma16(a, frag[(n * KT + kt) * 64 + lane], acc);
+            }
+            const int ch = n * 16 + r;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int mm = t * 16 + kq + i;
+                const float g = gelu(acc[i] + bias[ch]);
+                if (mm < M) {
+                    if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
+                    else ((half_t*)outp)[ch * M + mm] = (half_t)g;
+                }
+            }
+        }
+        __syncthreads();
+        return;
+    }
 #pragma unroll
     for (int u = 0; u < TPW; u++) {
         const int t = wave + NW * u;
