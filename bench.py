@@ -597,13 +597,14 @@ def main():
         lat.append(time.perf_counter() - t0)
     lat = np.array(lat[20:]) * 1e3
 
+    shape = _lib.lib().drw_get_launch_shape  # the template arguments rocprofv3 names the kernel with
     if fused:
-        dom_kernel, dom_flop, dom_ms = "poisson_query_kernel", SEL_FLOP + PRED_FLOP, fused_ms
+        dom_kernel, dom_flop, dom_ms = f"poisson_query_kernel<{shape(3, B)}>", SEL_FLOP + PRED_FLOP, fused_ms
         dom_note = (f"algorithmic {SEL_FLOP + PRED_FLOP:,} FLOP per query (PoissonSelector incl. its Linear head "
                     f"{SEL_FLOP:,} + PoissonPredictor {PRED_FLOP:,}); the kernel is bound by VALU issue "
                     "(depthwise conv + GELU + voxel lookups), see profiles/pmc_poisson_n23_b8192.md")
     else:
-        dom_kernel, dom_flop, dom_ms = "face_query_kernel<0>", PRED_FLOP, face_ms
+        dom_kernel, dom_flop, dom_ms = f"face_query_kernel<0, {shape(0, B)}>", PRED_FLOP, face_ms
         dom_note = (f"algorithmic {PRED_FLOP:,} FLOP per query; the kernel is bound by VALU issue "
                     "(depthwise conv + GELU), see profiles/pmc_poisson_n23_b8192.md")
     achieved = B * dom_flop / (dom_ms * 1e-3) / 1e12

@@ -170,3 +170,31 @@ def test_resident_closed_loop_rate():
         if resident:
             print(f"  resident counters {sv.resident_stats()}", flush=True)
         sv.close()
+
+
+def test_resident_network_failure_goes_to_fdm(tmp_path):
+    """Models whose selector head biases are NaN make every resident draw
+    fail: the kernel answers with status 1, the dispatcher re-runs each query
+    on the FDM route at N = 23 with the same counter, and every result equals
+    a model-less server's FDM result for that counter (both kinds)."""
+    from deeprwcap_amd import weights as W
+    from deeprwcap_amd.serving import Server
+    states = {n: W.read_drwnet(os.path.join(ROOT, "models", f"{n}.drwnet")) for n in W.MODEL_ORDER}
+    for sel in ("PoissonSelector", "GradientSelectorWeight"):
+        last = max(k for k in states[sel] if k.startswith("head.") and k.endswith(".bias"))
+        states[sel][last] = np.full_like(states[sel][last], np.nan)
+    broken = W.export_model_dir(str(tmp_path / "broken"), states)
+    reqs = _requests(96, 41)
+    fdm = Server(0, None, max_batch=64)
+    want = _serve_many(fdm, reqs)
+    fdm.close()
+    print(f"resident network failures: {len(reqs)} queries on NaN-bias selectors", flush=True)
+    sv = Server(0, broken, resident=True)
+    got = _serve_many(sv, reqs)
+    rs = sv.resident_stats()
+    sv.close()
+    print(f"  resident counters {rs}", flush=True)
+    for i, r in enumerate(reqs):
+        w = 3 if r[0] == 0 else 5
+        assert np.isfinite(got[i, :w]).all(), f"query {i} {r[:3]}"
+        np.testing.assert_array_equal(got[i, :w], want[i, :w], err_msg=f"query {i} {r[:3]}")
