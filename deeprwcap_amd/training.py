@@ -393,6 +393,136 @@ def evaluate_face_l1(name, model, cubes, G, Gz, idx, dev="cuda", weighted=False)
     return float(np.mean(errs))
 
 
+def full_kernel_l1(sel, pred, x, G, rot):
+    """Full-kernel L1 of the Poisson query for cubes x [b,23,23,23]: the
+    selector's face probabilities times the predictor's normalised kernel on
+    each face's rotated cube, against the FDM kernel G [b,6,529], as
+    sum |G_nn - G| / sum |G| per cube (the accuracy gate's measure).  `rot`:
+    the six Poisson rotation tables [6, 23^3].  Returns (per-cube L1, G_nn,
+    raw predictor output [6b, 529])."""
+    b = x.shape[0]
+    p = F.softmax(sel(x), -1)                                          # [b, 6]
+    flat = x.reshape(b, -1)
+    xr = torch.stack([flat[:, rot[f]] for f in range(6)], 1).reshape(6 * b, *x.shape[1:])
+    y = pred(xr).reshape(b, 6, -1)
+    k = F.relu(y) + 1e-10
+    k = k / k.sum(-1, keepdim=True)
+    Gn = p[:, :, None] * k
+    return (Gn - G).abs().sum((1, 2)) / G.abs().sum((1, 2)), Gn, y
+
+
+def train_joint_poisson(cubes, G, sel_init, pred_init, minutes=10.0, batch=64, lr_sel=2e-4, lr_pred=2e-4,
+                        dev="cuda", log=print, seed=0, eval_every=250, n_eval=2048, ema=0.999, out_dir=None):
+    """Fine-tune the PoissonSelector and PoissonPredictor together on the
+    quantity the accuracy gate measures: per cube, the full 6x529 kernel
+    p_f * k_f of the two networks against the normalised FDM kernel, with
+    loss L1 + 3174 * MSE (plus the predictor's relu-head revival term).  Each
+    step runs the selector on `batch` cubes and the predictor on all six
+    rotated views of each (batch-norm statistics of the six-face mix the
+    running statistics describe at inference).  Both networks start from the
+    given .drwnet files; the held-out full-kernel L1 of the starting pair is
+    measured first and a pair is kept only when it beats it.  With
+    `out_dir`, every new best pair is written there as
+    PoissonSelector.drwnet and PoissonPredictor.drwnet.  Returns (best
+    held-out L1, starting held-out L1)."""
+    torch.manual_seed(seed)
+    n = cubes.shape[0]
+    ntr = int(n * 0.95)
+    Xall = torch.from_numpy(cubes).to(dev)
+    Gp = torch.from_numpy(G).to(dev)
+    sel, pred = SelectorNet().to(dev), FaceNet().to(dev)
+    for m, path in ((sel, sel_init), (pred, pred_init)):
+        st = {k: torch.from_numpy(v) for k, v in W.read_drwnet(path).items()}
+        missing, unexpected = m.load_state_dict(st, strict=False)
+        if unexpected or any(not k.endswith("num_batches_tracked") for k in missing):
+            raise ValueError(f"{path}: missing {missing}, unexpected {unexpected}")
+    rot = torch.stack([_rot_table(f, "poisson") for f in range(6)]).to(dev)
+    opt = torch.optim.AdamW([{"params": sel.parameters(), "lr": lr_sel, "base": lr_sel},
+                             {"params": pred.parameters(), "lr": lr_pred, "base": lr_pred}], weight_decay=1e-10)
+    avgs = [torch.optim.swa_utils.AveragedModel(m, multi_avg_fn=torch.optim.swa_utils.get_ema_multi_avg_fn(ema),
+                                                use_buffers=True) if ema > 0 else None for m in (sel, pred)]
+    ev = torch.arange(ntr, min(n, ntr + n_eval), device=dev)
+
+    def current():
+        return [a.module if a is not None else m for a, m in zip(avgs, (sel, pred))]
+
+    def held_out():
+        s, q = current()
+        s.eval()
+        q.eval()
+        errs = []
+        with torch.no_grad():
+            for i in range(0, ev.shape[0], 256):
+                j = ev[i:i + 256]
+                errs.append(full_kernel_l1(s, q, Xall[j].float(), Gp[j], rot)[0])
+        s.train()
+        q.train()
+        return torch.cat(errs).mean().item()
+
+    def save(states):
+        if out_dir:
+            os.makedirs(out_dir, exist_ok=True)
+            for name, st in zip(("PoissonSelector", "PoissonPredictor"), states):
+                W.write_drwnet(os.path.join(out_dir, f"{name}.drwnet"),
+                               {k: v.float().cpu().numpy() for k, v in st.items()
+                                if not k.endswith("num_batches_tracked")})
+
+    def step(st, total):
+        idx = torch.randint(0, ntr, (batch,), device=dev)
+        g = Gp[idx]
+        g = g / g.sum((1, 2), keepdim=True)
+        l1, Gn, y = full_kernel_l1(sel, pred, Xall[idx].float(), g, rot)
+        revive = F.relu(-y).mean() / (y.abs().mean().detach() + 1e-6)
+        loss = l1.mean() + 3174.0 * F.mse_loss(Gn, g) + 0.1 * revive
+        opt.zero_grad(set_to_none=True)
+        loss.backward()
+        torch.nn.utils.clip_grad_norm_(list(sel.parameters()) + list(pred.parameters()), 1.0)
+        warm = max(1, total // 20)
+        frac = st / warm if st < warm else 0.5 * (1 + math.cos(math.pi * (st - warm) / max(1, total - warm)))
+        for gr in opt.param_groups:
+            gr["lr"] = gr["base"] * max(frac, 0.02)
+        opt.step()
+        for a, m in zip(avgs, (sel, pred)):
+            if a is not None:
+                a.update_parameters(m)
+        return loss
+
+    start = held_out()
+    best = start
+    best_states = [{k: v.detach().clone() for k, v in m.state_dict().items()} for m in (sel, pred)]
+    log(f"joint Poisson: starting pair held-out full-kernel L1 {start:.5f} ({ev.shape[0]} cubes)")
+    t0 = time.time()
+    for st in range(20):
+        step(st, 1000)
+    if dev != "cpu":
+        torch.cuda.synchronize()
+    t1 = time.time()
+    for st in range(20):
+        step(st, 1000)
+    if dev != "cpu":
+        torch.cuda.synchronize()
+    per = (time.time() - t1) / 20
+    total = max(50, int((minutes * 60 - (time.time() - t0)) / (1.2 * per)))
+    log(f"joint Poisson: {per * 1e3:.1f} ms/step at batch {batch} -> {total} steps")
+    tot_l, nb = 0.0, 0
+    for st in range(total):
+        loss = step(st, total)
+        if st % 50 == 0:
+            tot_l += loss.item()
+            nb += 1
+        if (st + 1) % eval_every == 0 or st == total - 1:
+            e = held_out()
+            if e < best:
+                best = e
+                best_states = [{k: v.detach().clone() for k, v in m.state_dict().items()} for m in current()]
+                save(best_states)
+            log(f"joint step {st + 1}/{total} loss {tot_l / max(nb, 1):.5f} held-out {e:.5f} (best {best:.5f}, "
+                f"start {start:.5f}, {time.time() - t0:.0f}s)")
+            tot_l, nb = 0.0, 0
+    save(best_states)
+    return best, start
+
+
 # --------------------------------------------------------------------------
 # GGFT dataset files (poisson.bin / gradient.bin, the reference's training
 # input): a float64 header [N, block_width], then per sample the N^3
@@ -572,6 +702,9 @@ if __name__ == "__main__":
     ap.add_argument("--only", help="train this network only (needs --data)")
     ap.add_argument("--init", metavar="DRWNET", help="with --only: start from these weights")
     ap.add_argument("--parallel", action="store_true", help="one process per network (needs --data)")
+    ap.add_argument("--joint", nargs=2, metavar=("SELECTOR", "PREDICTOR"),
+                    help="fine-tune a PoissonSelector and PoissonPredictor (.drwnet) together on the full-kernel "
+                         "L1 (needs --data; --lr for both, --batch cubes per step)")
     args = ap.parse_args()
 
     def _log(msg):
@@ -585,6 +718,12 @@ if __name__ == "__main__":
         t0 = time.time()
         save_dataset(args.make_data, *fdm_dataset(args.n, args.seed))
         _log(f"dataset: {args.n} cubes in {time.time() - t0:.1f}s -> {args.make_data}.*.npy")
+    elif args.joint:
+        cubes, G, _ = load_dataset(args.data)
+        best, start = train_joint_poisson(cubes, G, *args.joint, minutes=args.minutes or 10.0, batch=args.batch,
+                                          lr_sel=args.lr, lr_pred=args.lr, log=_log, seed=args.seed, ema=args.ema,
+                                          out_dir=args.out)
+        _log(f"joint Poisson: held-out full-kernel L1 {start:.5f} -> {best:.5f}")
     elif args.only:
         train_one(args.out, args.only, *load_dataset(args.data), args.minutes, args.batch, args.lr, args.seed, _log,
                   loss_kind=args.loss, ema=args.ema, init=args.init, face_sampling=args.face_sampling)

@@ -42,3 +42,60 @@ def test_weighted_held_out_is_the_probability_weighted_face_l1(orc):
     assert abs(got_u - l1.mean()) < 1e-5 + 1e-6
     # the two differ whenever the faces' errors and probabilities differ
     assert abs(want_w - l1.mean()) > 1e-6
+
+
+def _oracle_case(orc, n, seed):
+    from deeprwcap_amd import structures as S
+    L, C, _ = S.layered_boxes(n, 23, seed=seed)
+    cubes, _ = orc.voxelize(L, C, 23, normalise=True)
+    raw, _ = orc.voxelize(L, C, 23, normalise=False)
+    G = np.stack([orc.fdm_solve(raw[b].astype(np.float64), orc.GFT, tol=1e-9)[0].reshape(6, 529)
+                  for b in range(n)]).astype(np.float32)
+    return L, C, cubes.astype(np.float32), G
+
+
+def test_joint_loss_is_the_full_kernel_l1(orc):
+    """full_kernel_l1 (the joint fine-tuning's loss and held-out error) equals
+    the gate's sum |p_f k_f - G| / sum |G| computed from the FP32 restatement
+    of the whole Poisson query (its own voxelizer, rotations and heads)."""
+    import torch
+    from deeprwcap_amd import training as T
+    from deeprwcap_amd import weights as W
+    from oracle import nets_ref as R
+    n = 5
+    L, C, cubes, G = _oracle_case(orc, n, 654)
+    mdir = os.path.join(ROOT, "models")
+    params = {m: {k: torch.from_numpy(v) for k, v in W.read_drwnet(os.path.join(mdir, f"{m}.drwnet")).items()}
+              for m in ("PoissonSelector", "PoissonPredictor")}
+    sel, pred = T.SelectorNet(), T.FaceNet()
+    sel.load_state_dict(params["PoissonSelector"], strict=False)
+    pred.load_state_dict(params["PoissonPredictor"], strict=False)
+    sel.eval()
+    pred.eval()
+    rot = torch.stack([T._rot_table(f, "poisson") for f in range(6)])
+    with torch.no_grad():
+        got, Gn, _ = T.full_kernel_l1(sel, pred, torch.from_numpy(cubes), torch.from_numpy(G), rot)
+    probs, _, rk = R.query_kernels(0, L, C, None, params)
+    want = np.abs(probs[:, :, None] * rk - G).sum((1, 2)) / np.abs(G).sum((1, 2))
+    np.testing.assert_allclose(got.numpy(), want, rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(Gn.numpy(), probs[:, :, None] * rk, rtol=1e-4, atol=1e-7)
+
+
+def test_joint_fine_tuning_keeps_the_better_pair(orc, tmp_path):
+    """A short joint run on the CPU: it measures the starting pair first,
+    returns a best held-out error no worse than it, and writes a pair of
+    .drwnet files that load into the reference module layout."""
+    from deeprwcap_amd import training as T
+    from deeprwcap_amd import weights as W
+    _, _, cubes, G = _oracle_case(orc, 24, 655)
+    mdir = os.path.join(ROOT, "models")
+    logs = []
+    best, start = T.train_joint_poisson(cubes.astype(np.float16), G, os.path.join(mdir, "PoissonSelector.drwnet"),
+                                        os.path.join(mdir, "PoissonPredictor.drwnet"), minutes=0.02, batch=4,
+                                        dev="cpu", log=logs.append, eval_every=10, n_eval=8, out_dir=str(tmp_path))
+    assert best <= start and start > 0
+    assert any("starting pair" in m for m in logs)
+    for name in ("PoissonSelector", "PoissonPredictor"):
+        st = W.read_drwnet(str(tmp_path / f"{name}.drwnet"))
+        ref = W.read_drwnet(os.path.join(mdir, f"{name}.drwnet"))
+        assert set(st) == set(ref)
