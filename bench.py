@@ -87,31 +87,39 @@ def time_steps(run, steps, warmup, torch):
     return time.perf_counter() - t0
 
 
+def fdm_iter_flop(N):
+    """FLOP of one inner FP32 CG iteration on an N^3 cube.  Odd N (red-black
+    reduced system): black half sweep 11 per black cell, red half sweep 12
+    and its dot 2, two axpy 4, norm 2 and the direction update 2 per red
+    cell, 16.5 N^3 in all.  Even N (Jacobi PCG): 7-point SpMV 13, two dots
+    4, three axpy 6, Jacobi 1, norm 1: 25 N^3."""
+    return (16.5 if N % 2 else 25.0) * float(N) ** 3
+
+
 def fdm_roofline(L, C, kernel_ms):
-    """Roofline of the batched FDM kernel on the same batch: the inner FP32
-    CG costs 25 N^3 FLOP per iteration (7-point SpMV 13, two dots 4, three
-    axpy 6, Jacobi 1, norm 1); the iteration counts K come from re-running
-    the (deterministic) solve with statistics on."""
+    """Roofline of the batched FDM kernel on the same batch: K inner CG
+    iterations of fdm_iter_flop(23) each per cube; the iteration counts K
+    come from re-running the (deterministic) solve with statistics on."""
     from deeprwcap_amd import device as D
     B, N = L.shape[0], 23
     pk = D.PackedCubes([N] * B)
     D.voxelize(L, C, fixed_N=N, normalise=False, out=pk.eps.view(B, N, N, N))
     iters, _ = D.fdm_solve(pk, 0, tol=1e-6)
     K = iters.cpu().numpy().astype(np.float64)
-    flop = 25.0 * N ** 3 * K.sum()
+    flop = fdm_iter_flop(N) * K.sum()
     achieved = flop / (kernel_ms * 1e-3) / 1e12
     return {"kernel": "fdm_pcg_kernel", "bound": "valu", "achieved": achieved, "peak": FP32_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": achieved / FP32_VALU_PEAK_TFLOPS, "iters_mean": float(K.mean()),
             "hbm_algorithmic_bytes": B * (4 * N ** 3 + 48 * N ** 2 + 16),
-            "note": "K*25*N^3 FLOP per solve at tol 1e-6, fp32 vector peak; the kernel is bound by LDS and "
-                    "barrier latency (one 1024-thread workgroup per CU)"}
+            "note": "K * 16.5 N^3 FLOP per solve at tol 1e-6 (red-black reduced-system CG), fp32 vector peak; "
+                    "the kernel is bound by LDS and barrier latency (one 1024-thread workgroup per CU)"}
 
 
 def fdm_batch(torch, B=4096, seed=0):
     """Config 2: B cubes with N uniform over {16, 21, 23} (1-3 planar layers
     + 0-2 boxes, eps U[1,8]) solved in one FDM launch at tol 1e-6 (GFT);
-    cubes/s, mean CG iterations and K*25*N^3 FLOP against the fp32 vector
-    peak."""
+    cubes/s, mean CG iterations and K * fdm_iter_flop(N) FLOP against the
+    fp32 vector peak."""
     from deeprwcap_amd import device as D, structures as S
     Ns = np.random.default_rng(seed).choice(np.array([16, 21, 23], np.int32), size=B)
     L, C, _ = S.layered_boxes(B, Ns, seed=seed)
@@ -127,7 +135,7 @@ def fdm_batch(torch, B=4096, seed=0):
         dts.append(time.perf_counter() - t0)
     dt = float(np.median(dts))
     K = iters.cpu().numpy().astype(np.float64)
-    flop = 25.0 * float((K * Ns.astype(np.float64) ** 3).sum())
+    flop = float(sum(k * fdm_iter_flop(int(n)) for k, n in zip(K, Ns)))
     achieved = flop / dt / 1e12
     return {"cubes": B, "N": "uniform {16, 21, 23}", "seconds": dt, "cubes_per_s": B / dt,
             "iters_mean": float(K.mean()), "achieved_tflops": achieved,

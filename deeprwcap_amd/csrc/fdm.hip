@@ -16,10 +16,11 @@
 //    (N^3 + 2 N^2 floats with zero guard planes, so the stencil reads need no
 //    bounds checks: x/y wrap-around reads hit a zero weight) and the Jacobi
 //    scales s_c; A p, x and r of the own cells stay in registers;
-//  * inner solver: FP32 CG on the scaled system (= Jacobi PCG), block-wide
-//    reductions by wave64 DPP sums + a 16-entry LDS table, summed in a
-//    fixed order so every thread sees bit-identical alpha/beta (held in
-//    scalar registers);
+//  * inner solver: FP32 CG on the Jacobi-scaled system; for odd N on its
+//    red-black reduced system (half the iterations, see fdm_cube), for even
+//    N on the full one (= Jacobi PCG); block-wide reductions by wave64 DPP
+//    sums + a 16-entry LDS table, summed in a fixed order so every thread
+//    sees bit-identical alpha/beta (held in scalar registers);
 //  * outer loop: FP64 iterative refinement.  The residual e - L y is formed
 //    in FP64 with exact harmonic means and the loop stops at ||e - L y|| <=
 //    tol ||e|| (the reference's relative-residual criterion,
@@ -182,7 +183,28 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     static_assert(N <= SH::kMaxN, "cube larger than the size class");
     constexpr int NN = N * N, NNN = N * NN;
     constexpr int T = SH::kThreads, W = SH::kWaves;
-    constexpr int CPT = (NNN + T - 1) / T;
+    // Odd N: the inner solve is CG on the red-black reduced system.  The
+    // 7-point grid is two-coloured by the parity of i + j + k, which for odd
+    // N is the parity of c = i + N j + N^2 k itself, and every neighbour of a
+    // cell has the other colour.  With the Jacobi-scaled operator A = I - B
+    // (B: the scaled couplings, red <-> black only), the black unknowns are
+    // x_b = b_b + B_br x_r, and the red ones solve the Schur complement
+    // (I - B_rb B_br) x_r = b_r + B_rb b_b, whose condition number is about a
+    // quarter of A's: CG needs half the iterations, each one a black half
+    // sweep and a red half sweep (the work of one stencil pass).  The full
+    // residual of (x_r, x_b) is the reduced one on the red cells and zero on
+    // the black ones, so the stopping test is unchanged.  A thread's slots
+    // q < CPH are red cells c = 2h, the others black cells c = 2h + 1, with
+    // h = tid + (q mod CPH) T.  Even N: Jacobi PCG on c = tid + q T.
+    constexpr bool RB = (N & 1) != 0;
+    constexpr int CPH = ((NNN + 1) / 2 + T - 1) / T;
+    constexpr int CPT = RB ? 2 * CPH : (NNN + T - 1) / T;
+    constexpr int STEP = RB ? 2 : 1;  // thread stride of the slot cells
+    // offset of slot q's cell from the thread's first cell STEP * tid
+    auto off = [](int q) __attribute__((always_inline)) { return RB ? 2 * (q % CPH) * T + (q >= CPH ? 1 : 0) : q * T; };
+    // only the last slot of each colour can lie past the last cell; the
+    // other slots' tests fold away after unrolling
+    auto edge = [](int q) __attribute__((always_inline)) { return RB ? (q == CPH - 1 || q == CPT - 1) : q == CPT - 1; };
     // A p stays in registers and the Jacobi scales s_c in the third LDS
     // array for the whole solve (s is read before and after each inner solve
     // only, so in registers it would be spilled across the loop).
@@ -217,9 +239,9 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     float sc[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-        const int c = tid + q * T;
+        const int c = STEP * tid + off(q);
         sc[q] = 0.f;
-        if (q < CPT - 1 || c < NNN) {
+        if (!edge(q) || c < NNN) {
             const int i = c % N, j = (c / N) % N, k = c / NN;
             const float ec = EL[c];
             float d = 0.f;
@@ -239,8 +261,8 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     float w[4][CPT];  // x-, x+, y-, y+
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-        const int c = tid + q * T;
-        if (q < CPT - 1 || c < NNN) {
+        const int c = STEP * tid + off(q);
+        if (!edge(q) || c < NNN) {
             const int i = c % N, j = (c / N) % N, k = c / NN;
             const float ec = EL[c], sc = P[c];
             w[0][q] = i > 0 ? harm(ec, EL[c - 1]) * sc * P[c - 1] : 0.f;
@@ -277,7 +299,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             // ---- FP64 residual norm of y = yscale S x --------------------------
             double loc = 0.0;
             const float* Ev = E;  // address arithmetic kept inside the pass (not held, and spilled, across rounds)
-            asm volatile("" : "+s"(Ev));
+            asm volatile("" : "+v"(Ev));
 #pragma unroll 1
             for (int c = tid; c < NNN; c += T) {
                 const int i = c % N, j = (c / N) % N, k = c / NN;
@@ -304,7 +326,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             }
             // a further round: y goes to Y, and the residual is formed from it
             double* Yv = a.Y;
-            asm volatile("" : "+s"(Yv));  // addresses of the rare path stay in it
+            asm volatile("" : "+v"(Yv));  // addresses of the rare path stay in it
             for (int c = tid; c < NNN; c += T) Yv[c] = (double)SV[c] * (double)XV[c] * yscale;
             implicit = false;
             __syncthreads();
@@ -323,7 +345,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
         } else {
             const double* Yv = a.Y;
             const float* Ev = E;
-            asm volatile("" : "+s"(Yv), "+s"(Ev));
+            asm volatile("" : "+v"(Yv), "+v"(Ev));
 #pragma unroll 1
             for (int c = tid; c < NNN; c += T) {
                 const int i = c % N, j = (c / N) % N, k = c / NN;
@@ -356,72 +378,163 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
         if (tgt < (double)a.inner_floor) tgt = a.inner_floor;
         const float tgt2 = (float)(tgt * tgt);
         float xh[CPT];
-        // Per-thread LDS bases; element q*T + k of each is cell c = tid + q*T
+        // Per-thread LDS bases; element off(q) + k of each is slot q's cell
         // shifted by k - N^2.
-        lds_float* Pb = lds_base(sh.p + tid);
-        lds_float* Zb = lds_base(sh.wz + tid);
+        lds_float* Pb = lds_base(sh.p + STEP * tid);
+        lds_float* Zb = lds_base(sh.wz + STEP * tid);
+        // sum over the six neighbours n of slot q's cell of w_cn P[n]
+        auto nbsum = [&](int q) __attribute__((always_inline)) {
+            const int o = off(q) + NN;
+            float v = w[0][q] * Pb[o - 1];
+            v += w[1][q] * Pb[o + 1];
+            v += w[2][q] * Pb[o - N];
+            v += w[3][q] * Pb[o + N];
+            v += Zb[o - NN] * Pb[o - NN];
+            v += Zb[o] * Pb[o + NN];
+            return v;
+        };
         {
             float rh[CPT];
             float rloc = 0.f;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
+                const int c = STEP * tid + off(q);
                 xh[q] = 0.f;
-                rh[q] = (q < CPT - 1 || c < NNN) ? R32[c] * SC[c] * inv : 0.f;
+                rh[q] = (!edge(q) || c < NNN) ? R32[c] * SC[c] * inv : 0.f;
                 rloc += rh[q] * rh[q];
             }
             float rr = block_sum<W>(rloc, sh.red[1]);
             const float thresh = tgt2 * rr;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
-                if (q < CPT - 1 || c < NNN) P[c] = rh[q];
+                const int c = STEP * tid + off(q);
+                if (!edge(q) || c < NNN) P[c] = rh[q];
             }
             __syncthreads();
-            for (int it = 0; it < a.max_inner; it++) {
-                float pl = 0.f;
-                float apv[CPT];
-#pragma unroll
-                for (int q = 0; q < CPT; q++) {
-                    const int c = tid + q * T;
-                    const int o = q * T + NN;  // P index of cell c relative to Pb
-                    apv[q] = 0.f;
-                    if (q < CPT - 1 || c < NNN) {
-                        const float pc = Pb[o];
-                        float v = pc;
-                        v -= w[0][q] * Pb[o - 1];
-                        v -= w[1][q] * Pb[o + 1];
-                        v -= w[2][q] * Pb[o - N];
-                        v -= w[3][q] * Pb[o + N];
-                        v -= Zb[q * T] * Pb[o - NN];
-                        v -= Zb[q * T + NN] * Pb[o + NN];
-                        apv[q] = v;
-                        pl += pc * v;
-                    }
-                }
-                const float pq = block_sum<W>(pl, sh.red[0]);
-                const float alpha = rr / pq;
+            if constexpr (RB) {
+                // reduced right-hand side b_r + B_rb b_b on the red slots (rh of
+                // the black slots keeps b_b for the recovery below)
                 float rl = 0.f;
 #pragma unroll
-                for (int q = 0; q < CPT; q++) {
-                    const int c = tid + q * T;
-                    if (q < CPT - 1 || c < NNN) {
-                        xh[q] += alpha * Pb[q * T + NN];
-                        rh[q] -= alpha * apv[q];
+                for (int q = 0; q < CPH; q++) {
+                    const int c = STEP * tid + off(q);
+                    if (!edge(q) || c < NNN) {
+                        rh[q] += nbsum(q);
                         rl += rh[q] * rh[q];
                     }
                 }
-                const float rr_new = block_sum<W>(rl, sh.red[1]);
-                total_it++;
-                if (!(rr_new > thresh)) break;
-                const float beta = rr_new / rr;
-                rr = rr_new;
+                rr = block_sum<W>(rl, sh.red[1]);  // also: every read of b is done
+                float pv[CPH];
 #pragma unroll
-                for (int q = 0; q < CPT; q++) {
-                    const int c = tid + q * T;
-                    if (q < CPT - 1 || c < NNN) Pb[q * T + NN] = rh[q] + beta * Pb[q * T + NN];
+                for (int q = 0; q < CPH; q++) pv[q] = rh[q];
+                if (rr > thresh) {
+                    for (int it = 0; it < a.max_inner; it++) {
+#pragma unroll
+                        for (int q = 0; q < CPH; q++) {
+                            const int c = STEP * tid + off(q);
+                            if (!edge(q) || c < NNN) Pb[off(q) + NN] = pv[q];
+                        }
+                        __syncthreads();
+                        // black half sweep: t_b = B_br p
+#pragma unroll
+                        for (int q = CPH; q < CPT; q++) {
+                            const int c = STEP * tid + off(q);
+                            if (!edge(q) || c < NNN) Pb[off(q) + NN] = nbsum(q);
+                        }
+                        __syncthreads();
+                        // red half sweep: (I - B_rb B_br) p = p - B_rb t
+                        float pl = 0.f;
+                        float sp[CPH];
+#pragma unroll
+                        for (int q = 0; q < CPH; q++) {
+                            const int c = STEP * tid + off(q);
+                            sp[q] = 0.f;
+                            if (!edge(q) || c < NNN) {
+                                sp[q] = pv[q] - nbsum(q);
+                                pl += pv[q] * sp[q];
+                            }
+                        }
+                        const float pq = block_sum<W>(pl, sh.red[0]);
+                        const float alpha = rr / pq;
+                        float rl2 = 0.f;
+#pragma unroll
+                        for (int q = 0; q < CPH; q++) {
+                            const int c = STEP * tid + off(q);
+                            if (!edge(q) || c < NNN) {
+                                xh[q] += alpha * pv[q];
+                                rh[q] -= alpha * sp[q];
+                                rl2 += rh[q] * rh[q];
+                            }
+                        }
+                        const float rr_new = block_sum<W>(rl2, sh.red[1]);
+                        total_it++;
+                        if (!(rr_new > thresh)) break;
+                        const float beta = rr_new / rr;
+                        rr = rr_new;
+#pragma unroll
+                        for (int q = 0; q < CPH; q++) pv[q] = rh[q] + beta * pv[q];
+                    }
+                }
+                // black unknowns x_b = b_b + B_br x_r (the last reduction above,
+                // or the one before the loop, ordered every earlier read of P)
+#pragma unroll
+                for (int q = 0; q < CPH; q++) {
+                    const int c = STEP * tid + off(q);
+                    if (!edge(q) || c < NNN) Pb[off(q) + NN] = xh[q];
                 }
                 __syncthreads();
+#pragma unroll
+                for (int q = CPH; q < CPT; q++) {
+                    const int c = STEP * tid + off(q);
+                    if (!edge(q) || c < NNN) xh[q] = rh[q] + nbsum(q);
+                }
+                __syncthreads();  // every read of x_r done before P is rewritten
+            } else {
+                for (int it = 0; it < a.max_inner; it++) {
+                    float pl = 0.f;
+                    float apv[CPT];
+#pragma unroll
+                    for (int q = 0; q < CPT; q++) {
+                        const int c = STEP * tid + off(q);
+                        apv[q] = 0.f;
+                        if (!edge(q) || c < NNN) {
+                            const int o = off(q) + NN;  // P index of cell c relative to Pb
+                            const float pc = Pb[o];
+                            float v = pc;
+                            v -= w[0][q] * Pb[o - 1];
+                            v -= w[1][q] * Pb[o + 1];
+                            v -= w[2][q] * Pb[o - N];
+                            v -= w[3][q] * Pb[o + N];
+                            v -= Zb[o - NN] * Pb[o - NN];
+                            v -= Zb[o] * Pb[o + NN];
+                            apv[q] = v;
+                            pl += pc * v;
+                        }
+                    }
+                    const float pq = block_sum<W>(pl, sh.red[0]);
+                    const float alpha = rr / pq;
+                    float rl = 0.f;
+#pragma unroll
+                    for (int q = 0; q < CPT; q++) {
+                        const int c = STEP * tid + off(q);
+                        if (!edge(q) || c < NNN) {
+                            xh[q] += alpha * Pb[off(q) + NN];
+                            rh[q] -= alpha * apv[q];
+                            rl += rh[q] * rh[q];
+                        }
+                    }
+                    const float rr_new = block_sum<W>(rl, sh.red[1]);
+                    total_it++;
+                    if (!(rr_new > thresh)) break;
+                    const float beta = rr_new / rr;
+                    rr = rr_new;
+#pragma unroll
+                    for (int q = 0; q < CPT; q++) {
+                        const int c = STEP * tid + off(q);
+                        if (!edge(q) || c < NNN) Pb[off(q) + NN] = rh[q] + beta * Pb[off(q) + NN];
+                    }
+                    __syncthreads();
+                }
             }
         }
         // ---- y += ||r|| S x (implicit after the first solve) -------------------
@@ -432,16 +545,16 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             yscale = rnorm;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
-                if (q < CPT - 1 || c < NNN) XV[c] = xh[q];
+                const int c = STEP * tid + off(q);
+                if (!edge(q) || c < NNN) XV[c] = xh[q];
             }
         } else {
             double* Yv = a.Y;
-            asm volatile("" : "+s"(Yv));
+            asm volatile("" : "+v"(Yv));
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = tid + q * T;
-                if (q < CPT - 1 || c < NNN) Yv[c] += (double)SC[c] * (double)xh[q] * rnorm;
+                const int c = STEP * tid + off(q);
+                if (!edge(q) || c < NNN) Yv[c] += (double)SC[c] * (double)xh[q] * rnorm;
             }
         }
         have_y = true;
@@ -451,7 +564,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     // ---- G_b = 2 eps_c y_c, GGFT panel order (ggft_solver.h:65-77) ----------
     double* G = a.G;
     const float* Ev = E;
-    asm volatile("" : "+s"(G), "+s"(Ev));
+    asm volatile("" : "+v"(G), "+v"(Ev));
     if (implicit) {
         // each boundary cell's owner writes its panels (a cell on an edge or a
         // corner lies on two or three)
@@ -470,7 +583,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
         }
     } else {
         const double* Yv = a.Y;
-        asm volatile("" : "+s"(Yv));
+        asm volatile("" : "+v"(Yv));
         for (int pb = tid; pb < 6 * NN; pb += T) {
             const int f = pb / NN, t = pb - f * NN, u = t % N, v = t / N;
             int c;

@@ -2,7 +2,7 @@
 
 Compares the HIP path with the oracle / golden fixtures on the same inputs:
 * GGFT golden vectors (reference solver built from its sources, N = 16, 21,
-  23, all four right-hand sides): FP64 mode (tol 1e-12) within 1e-10
+  23, all four right-hand sides): FP64 mode (tol 1e-13, the golden vectors' own) within 1e-10
   relative, refinement to 1e-8 within 1e-6;
 * mixed N in {8, 9, 16, 21, 23} and all four right-hand sides against the
   FP64 oracle;
@@ -41,7 +41,10 @@ def test_fdm_golden_ggft(dev, orc, golden_ggft_n):
     N = int(g["N"])
     cubes, _ = orc.voxelize(g["layers"], g["cuboids"], N, normalise=False)
     B = len(g["codes"])
-    for tol, bound in ((1e-12, 1e-10), (1e-8, 1e-6)):
+    # at the golden vectors' own tolerance (1e-13) the solve matches them to
+    # 1e-10; at 1e-8 to 1e-6 (the forward error is the relative residual times
+    # a conditioning factor of up to a few hundred at these sizes)
+    for tol, bound in ((1e-13, 1e-10), (1e-8, 1e-6)):
         pk, G, it, rr = _solve(dev, list(cubes), [N] * B, g["codes"], tol)
         for b in range(B):
             k = G[pk.G_off[b]:pk.G_off[b] + 6 * N * N]
@@ -57,7 +60,7 @@ def test_fdm_mixed_n_vs_oracle(dev, orc):
     codes = np.arange(len(Ns), dtype=np.int32) % 4
     L, C, _ = S.layered_boxes(len(Ns), Ns, seed=7)
     cubes = [orc.voxelize(L[b:b + 1], C[b:b + 1], int(Ns[b]), normalise=False)[0][0] for b in range(len(Ns))]
-    pk, G, it, rr = _solve(dev, cubes, Ns, codes, 1e-12)
+    pk, G, it, rr = _solve(dev, cubes, Ns, codes, 1e-13)
     for b in range(len(Ns)):
         Gr, _, _ = orc.fdm_solve(cubes[b].astype(np.float64), int(codes[b]), tol=1e-14)
         k = G[pk.G_off[b]:pk.G_off[b] + Gr.size]
@@ -65,7 +68,7 @@ def test_fdm_mixed_n_vs_oracle(dev, orc):
         assert _relerr(k, Gr) < 1e-10
 
 
-@pytest.mark.parametrize("tol,bound", [(1e-12, 1e-10), (1e-5, 5e-3)])
+@pytest.mark.parametrize("tol,bound", [(1e-13, 1e-10), (1e-5, 5e-3)])
 def test_fdm_layer_only_cubes(dev, orc, tol, bound):
     """Cubes described by planar z-layers only, as a walker issues them:
     homogeneous, an interface on the centre plane, off-grid interfaces;
