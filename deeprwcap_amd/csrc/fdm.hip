@@ -205,6 +205,19 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     // only the last slot of each colour can lie past the last cell; the
     // other slots' tests fold away after unrolling
     auto edge = [](int q) __attribute__((always_inline)) { return RB ? (q == CPH - 1 || q == CPT - 1) : q == CPT - 1; };
+    // Odd N, colour-split LDS layout of the inner solve: red cell 2h at
+    // R[h] = p[R0 + h], black cell 2h + 1 at K[h] = p[K0 + h], and their z+
+    // couplings at ZR[h] = wz[ZR0 + h], ZK[h] = wz[ZK0 + h].  Every neighbour
+    // of a colour sits at a constant offset in the other colour's array
+    // (red h: black h - 1, h, h - NH - 1, h + NH, h -+ (NNH + 1 / 0); black h:
+    // red h, h + 1, h - NH, h + NH + 1, h - NNH, h + NNH + 1), so a lane's
+    // reads are consecutive words across the wave.  Guard entries (zero)
+    // cover the out-of-cube neighbours, whose couplings are zero.
+    constexpr int HR = (NNN + 1) / 2, HB = NNN / 2, NH = (N - 1) / 2, NNH = (NN - 1) / 2;
+    constexpr int R0 = NNH, K0 = HR + 3 * NNH + 1;        // split p: [0, NNN + 2 NN)
+    constexpr int ZR0 = NNH, ZK0 = HR + 2 * NNH + 1;      // split wz: [0, NNN + NN)
+    static_assert(!RB || K0 + HR + NNH <= SH::kCells + 2 * SH::kGuard, "split p exceeds its array");
+    static_assert(!RB || ZK0 + HB <= SH::kCells + SH::kGuard, "split wz exceeds its array");
     // A p stays in registers and the Jacobi scales s_c in the third LDS
     // array for the whole solve (s is read before and after each inner solve
     // only, so in registers it would be spilled across the loop).
@@ -226,6 +239,11 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     float* EL = sh.ap;       // eps staging during the set-up
     for (int c = tid; c < NNN; c += T) EL[c] = E[c];
     for (int g = tid; g < NN; g += T) { P[g - NN] = 0.f; P[NNN + g] = 0.f; WZ[g - NN] = 0.f; }
+    if constexpr (RB) {
+        // split wz guards: below ZR (black z- reads) and below ZK (red z- reads)
+        for (int g = tid; g < NNH; g += T) sh.wz[ZR0 - NNH + g] = 0.f;
+        for (int g = tid; g <= NNH; g += T) sh.wz[ZK0 - NNH - 1 + g] = 0.f;
+    }
     if (tid == 0) {
         fdm_rhs<N>(E, code, sh.rhs_idx, sh.rhs_val);
         double s = 0.0;
@@ -269,7 +287,9 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             w[1][q] = i < N - 1 ? harm(ec, EL[c + 1]) * sc * P[c + 1] : 0.f;
             w[2][q] = j > 0 ? harm(ec, EL[c - N]) * sc * P[c - N] : 0.f;
             w[3][q] = j < N - 1 ? harm(ec, EL[c + N]) * sc * P[c + N] : 0.f;
-            WZ[c] = k < N - 1 ? harm(ec, EL[c + NN]) * sc * P[c + NN] : 0.f;
+            const float wzc = k < N - 1 ? harm(ec, EL[c + NN]) * sc * P[c + NN] : 0.f;
+            if constexpr (RB) sh.wz[((c & 1) ? ZK0 : ZR0) + (c >> 1)] = wzc;
+            else WZ[c] = wzc;
         } else {
 #pragma unroll
             for (int d = 0; d < 4; d++) w[d][q] = 0.f;
@@ -378,21 +398,6 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
         if (tgt < (double)a.inner_floor) tgt = a.inner_floor;
         const float tgt2 = (float)(tgt * tgt);
         float xh[CPT];
-        // Per-thread LDS bases; element off(q) + k of each is slot q's cell
-        // shifted by k - N^2.
-        lds_float* Pb = lds_base(sh.p + STEP * tid);
-        lds_float* Zb = lds_base(sh.wz + STEP * tid);
-        // sum over the six neighbours n of slot q's cell of w_cn P[n]
-        auto nbsum = [&](int q) __attribute__((always_inline)) {
-            const int o = off(q) + NN;
-            float v = w[0][q] * Pb[o - 1];
-            v += w[1][q] * Pb[o + 1];
-            v += w[2][q] * Pb[o - N];
-            v += w[3][q] * Pb[o + N];
-            v += Zb[o - NN] * Pb[o - NN];
-            v += Zb[o] * Pb[o + NN];
-            return v;
-        };
         {
             float rh[CPT];
             float rloc = 0.f;
@@ -403,15 +408,57 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 rh[q] = (!edge(q) || c < NNN) ? R32[c] * SC[c] * inv : 0.f;
                 rloc += rh[q] * rh[q];
             }
-            float rr = block_sum<W>(rloc, sh.red[1]);
+            float rr = block_sum<W>(rloc, sh.red[1]);  // also: every read of R32 is done
             const float thresh = tgt2 * rr;
-#pragma unroll
-            for (int q = 0; q < CPT; q++) {
-                const int c = STEP * tid + off(q);
-                if (!edge(q) || c < NNN) P[c] = rh[q];
-            }
-            __syncthreads();
             if constexpr (RB) {
+                // Per-thread bases of the colour-split arrays; slot q < CPH is red
+                // h = tid + qT at R[qT], slot q >= CPH black h = tid + (q - CPH)T
+                // at K[(q - CPH)T].
+                lds_float* Rb = lds_base(sh.p + R0 + tid);
+                lds_float* Kb = lds_base(sh.p + K0 + tid);
+                lds_float* Zr = lds_base(sh.wz + ZR0 + tid);
+                lds_float* Zk = lds_base(sh.wz + ZK0 + tid);
+                // sum over the black neighbours of red slot q of w P
+                auto red_nb = [&](int q) __attribute__((always_inline)) {
+                    const int j = q * T;
+                    float v = w[0][q] * Kb[j - 1];
+                    v += w[1][q] * Kb[j];
+                    v += w[2][q] * Kb[j - NH - 1];
+                    v += w[3][q] * Kb[j + NH];
+                    v += Zk[j - NNH - 1] * Kb[j - NNH - 1];
+                    v += Zr[j] * Kb[j + NNH];
+                    return v;
+                };
+                // sum over the red neighbours of black slot q of w P
+                auto black_nb = [&](int q) __attribute__((always_inline)) {
+                    const int j = (q - CPH) * T;
+                    float v = w[0][q] * Rb[j];
+                    v += w[1][q] * Rb[j + 1];
+                    v += w[2][q] * Rb[j - NH];
+                    v += w[3][q] * Rb[j + NH + 1];
+                    v += Zr[j - NNH] * Rb[j - NNH];
+                    v += Zk[j] * Rb[j + NNH + 1];
+                    return v;
+                };
+                // guards of the split arrays (the p array held R32 until now)
+                for (int g = tid; g < NNH; g += T) {
+                    sh.p[R0 - NNH + g] = 0.f;
+                    sh.p[R0 + HR + g] = 0.f;
+                }
+                for (int g = tid; g <= NNH; g += T) {
+                    sh.p[K0 - NNH - 1 + g] = 0.f;
+                    sh.p[K0 + HB + g] = 0.f;
+                }
+                // b: red slots to R, black slots to K
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = STEP * tid + off(q);
+                    if (!edge(q) || c < NNN) {
+                        if (q < CPH) Rb[q * T] = rh[q];
+                        else Kb[(q - CPH) * T] = rh[q];
+                    }
+                }
+                __syncthreads();
                 // reduced right-hand side b_r + B_rb b_b on the red slots (rh of
                 // the black slots keeps b_b for the recovery below)
                 float rl = 0.f;
@@ -419,7 +466,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 for (int q = 0; q < CPH; q++) {
                     const int c = STEP * tid + off(q);
                     if (!edge(q) || c < NNN) {
-                        rh[q] += nbsum(q);
+                        rh[q] += red_nb(q);
                         rl += rh[q] * rh[q];
                     }
                 }
@@ -432,14 +479,14 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
 #pragma unroll
                         for (int q = 0; q < CPH; q++) {
                             const int c = STEP * tid + off(q);
-                            if (!edge(q) || c < NNN) Pb[off(q) + NN] = pv[q];
+                            if (!edge(q) || c < NNN) Rb[q * T] = pv[q];
                         }
                         __syncthreads();
                         // black half sweep: t_b = B_br p
 #pragma unroll
                         for (int q = CPH; q < CPT; q++) {
                             const int c = STEP * tid + off(q);
-                            if (!edge(q) || c < NNN) Pb[off(q) + NN] = nbsum(q);
+                            if (!edge(q) || c < NNN) Kb[(q - CPH) * T] = black_nb(q);
                         }
                         __syncthreads();
                         // red half sweep: (I - B_rb B_br) p = p - B_rb t
@@ -450,7 +497,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                             const int c = STEP * tid + off(q);
                             sp[q] = 0.f;
                             if (!edge(q) || c < NNN) {
-                                sp[q] = pv[q] - nbsum(q);
+                                sp[q] = pv[q] - red_nb(q);
                                 pl += pv[q] * sp[q];
                             }
                         }
@@ -476,20 +523,30 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                     }
                 }
                 // black unknowns x_b = b_b + B_br x_r (the last reduction above,
-                // or the one before the loop, ordered every earlier read of P)
+                // or the one before the loop, ordered every earlier read of R)
 #pragma unroll
                 for (int q = 0; q < CPH; q++) {
                     const int c = STEP * tid + off(q);
-                    if (!edge(q) || c < NNN) Pb[off(q) + NN] = xh[q];
+                    if (!edge(q) || c < NNN) Rb[q * T] = xh[q];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = CPH; q < CPT; q++) {
                     const int c = STEP * tid + off(q);
-                    if (!edge(q) || c < NNN) xh[q] = rh[q] + nbsum(q);
+                    if (!edge(q) || c < NNN) xh[q] = rh[q] + black_nb(q);
                 }
-                __syncthreads();  // every read of x_r done before P is rewritten
+                __syncthreads();  // every read of the split arrays done before P is rewritten
             } else {
+                // Per-thread LDS bases; element off(q) + k of each is slot q's
+                // cell shifted by k - N^2.
+                lds_float* Pb = lds_base(sh.p + tid);
+                lds_float* Zb = lds_base(sh.wz + tid);
+#pragma unroll
+                for (int q = 0; q < CPT; q++) {
+                    const int c = STEP * tid + off(q);
+                    if (!edge(q) || c < NNN) P[c] = rh[q];
+                }
+                __syncthreads();
                 for (int it = 0; it < a.max_inner; it++) {
                     float pl = 0.f;
                     float apv[CPT];
