@@ -88,12 +88,12 @@ def time_steps(run, steps, warmup, torch):
 
 
 def fdm_iter_flop(N):
-    """FLOP of one inner FP32 CG iteration on an N^3 cube.  Odd N (red-black
-    reduced system): black half sweep 11 per black cell, red half sweep 12
-    and its dot 2, two axpy 4, norm 2 and the direction update 2 per red
-    cell, 16.5 N^3 in all.  Even N (Jacobi PCG): 7-point SpMV 13, two dots
-    4, three axpy 6, Jacobi 1, norm 1: 25 N^3."""
-    return (16.5 if N % 2 else 25.0) * float(N) ** 3
+    """FLOP of one inner FP32 CG iteration on an N^3 cube (red-black reduced
+    system, every supported N): black half sweep 11 per black cell, red half
+    sweep 12 and its dot 2, two axpy 4, norm 2 and the direction update 2 per
+    red cell, 16.5 N^3 in all (a Jacobi-PCG iteration would be 25 N^3:
+    7-point SpMV 13, two dots 4, three axpy 6, Jacobi 1, norm 1)."""
+    return 16.5 * float(N) ** 3
 
 
 def fdm_roofline(L, C, kernel_ms):

@@ -9,18 +9,20 @@
 //
 // Design (persistent workgroups, one cube at a time each; 1024 threads and one
 // workgroup per CU up to N = 23, 256 threads and three per CU up to N = 16):
-//  * cells c = tid + q*T (q < 12 at N = 23); the x and y face couplings of
-//    each cell, symmetrically Jacobi-scaled (w_cn / sqrt(d_c d_n)), live in
-//    VGPRs for the whole solve; the z couplings live in LDS (the z- one of
-//    cell c is the z+ one of c - N^2), next to the search direction p
-//    (N^3 + 2 N^2 floats with zero guard planes, so the stencil reads need no
-//    bounds checks: x/y wrap-around reads hit a zero weight) and the Jacobi
-//    scales s_c; A p, x and r of the own cells stay in registers;
-//  * inner solver: FP32 CG on the Jacobi-scaled system; for odd N on its
-//    red-black reduced system (half the iterations, see fdm_cube), for even
-//    N on the full one (= Jacobi PCG); block-wide reductions by wave64 DPP
-//    sums + a 16-entry LDS table, summed in a fixed order so every thread
-//    sees bit-identical alpha/beta (held in scalar registers);
+//  * a thread owns the red and the black cell of x-pairs h = tid + q*T (12
+//    cells at N = 23); the x and y face couplings of each cell,
+//    symmetrically Jacobi-scaled (w_cn / sqrt(d_c d_n)), live in VGPRs for
+//    the whole solve; the z couplings live in LDS (the z- one of a cell is
+//    the z+ one of the cell below), next to the search direction p, with
+//    zero guards so the stencil reads need no bounds checks (x/y wrap-around
+//    reads hit a zero weight), and the Jacobi scales s_c; x, r and p of the
+//    own cells stay in registers;
+//  * inner solver: FP32 CG on the red-black reduced system of the
+//    Jacobi-scaled operator (half the iterations of Jacobi PCG, see
+//    fdm_cube), the two colours in separate LDS arrays; block-wide
+//    reductions by wave64 DPP sums + a 16-entry LDS table, summed in a
+//    fixed order so every thread sees bit-identical alpha/beta (held in
+//    scalar registers);
 //  * outer loop: FP64 iterative refinement.  The residual e - L y is formed
 //    in FP64 with exact harmonic means and the loop stops at ||e - L y|| <=
 //    tol ||e|| (the reference's relative-residual criterion,
@@ -183,40 +185,48 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     static_assert(N <= SH::kMaxN, "cube larger than the size class");
     constexpr int NN = N * N, NNN = N * NN;
     constexpr int T = SH::kThreads, W = SH::kWaves;
-    // Odd N: the inner solve is CG on the red-black reduced system.  The
-    // 7-point grid is two-coloured by the parity of i + j + k, which for odd
-    // N is the parity of c = i + N j + N^2 k itself, and every neighbour of a
-    // cell has the other colour.  With the Jacobi-scaled operator A = I - B
+    // The inner solve is CG on the red-black reduced system.  The 7-point
+    // grid is two-coloured by the parity of i + j + k, and every neighbour
+    // of a cell has the other colour.  With the Jacobi-scaled operator A = I - B
     // (B: the scaled couplings, red <-> black only), the black unknowns are
     // x_b = b_b + B_br x_r, and the red ones solve the Schur complement
     // (I - B_rb B_br) x_r = b_r + B_rb b_b, whose condition number is about a
     // quarter of A's: CG needs half the iterations, each one a black half
     // sweep and a red half sweep (the work of one stencil pass).  The full
     // residual of (x_r, x_b) is the reduced one on the red cells and zero on
-    // the black ones, so the stopping test is unchanged.  A thread's slots
-    // q < CPH are red cells c = 2h, the others black cells c = 2h + 1, with
-    // h = tid + (q mod CPH) T.  Even N: Jacobi PCG on c = tid + q T.
-    constexpr bool RB = (N & 1) != 0;
+    // the black ones, so the stopping test is unchanged.  Cells are taken in
+    // pairs h = (2h, 2h + 1) along x: for odd N the parity of c = i + N j +
+    // N^2 k is that of i + j + k, so pair h holds red 2h and black 2h + 1;
+    // for even N a row of N / 2 pairs starts with its red cell when j + k is
+    // even (par = 0) and with its black one otherwise (par = 1).  A thread's
+    // slots q < CPH are the red cells of pairs h = tid + (q mod CPH) T, the
+    // others the black cells of the same pairs.  For even N the row parity
+    // of a thread's pairs must not depend on q (T / (N/2) rows apart, a
+    // multiple of 2N: j unchanged, k by an even step); where it would, the
+    // kernel keeps Jacobi PCG on c = tid + q T.
+    constexpr bool RB = (N & 1) != 0 || (T % (N / 2) == 0 && (T / (N / 2)) % (2 * N) == 0);
     constexpr int CPH = ((NNN + 1) / 2 + T - 1) / T;
     constexpr int CPT = RB ? 2 * CPH : (NNN + T - 1) / T;
-    constexpr int STEP = RB ? 2 : 1;  // thread stride of the slot cells
-    // offset of slot q's cell from the thread's first cell STEP * tid
-    auto off = [](int q) __attribute__((always_inline)) { return RB ? 2 * (q % CPH) * T + (q >= CPH ? 1 : 0) : q * T; };
+    // Jacobi PCG: offset of slot q's cell from the thread's first cell tid
+    auto off = [](int q) __attribute__((always_inline)) { return q * T; };
     // only the last slot of each colour can lie past the last cell; the
     // other slots' tests fold away after unrolling
     auto edge = [](int q) __attribute__((always_inline)) { return RB ? (q == CPH - 1 || q == CPT - 1) : q == CPT - 1; };
-    // Odd N, colour-split LDS layout of the inner solve: red cell 2h at
-    // R[h] = p[R0 + h], black cell 2h + 1 at K[h] = p[K0 + h], and their z+
+    // Colour-split LDS layout of the inner solve: the red cell of pair h at
+    // R[h] = p[R0 + h], the black one at K[h] = p[K0 + h], and their z+
     // couplings at ZR[h] = wz[ZR0 + h], ZK[h] = wz[ZK0 + h].  Every neighbour
-    // of a colour sits at a constant offset in the other colour's array
-    // (red h: black h - 1, h, h - NH - 1, h + NH, h -+ (NNH + 1 / 0); black h:
-    // red h, h + 1, h - NH, h + NH + 1, h - NNH, h + NNH + 1), so a lane's
-    // reads are consecutive words across the wave.  Guard entries (zero)
-    // cover the out-of-cube neighbours, whose couplings are zero.
-    constexpr int HR = (NNN + 1) / 2, HB = NNN / 2, NH = (N - 1) / 2, NNH = (NN - 1) / 2;
-    constexpr int R0 = NNH, K0 = HR + 3 * NNH + 1;        // split p: [0, NNN + 2 NN)
-    constexpr int ZR0 = NNH, ZK0 = HR + 2 * NNH + 1;      // split wz: [0, NNN + NN)
-    static_assert(!RB || K0 + HR + NNH <= SH::kCells + 2 * SH::kGuard, "split p exceeds its array");
+    // of a colour sits at a fixed offset in the other colour's array: x at
+    // -1 + par, par (red) and -par, 1 - par (black), y at -(N/2) - (N&1),
+    // N/2 (red) and -(N/2), N/2 + (N&1) (black), z at -C, F (red) and -F, C
+    // (black), with F = floor(N^2/2), C = ceil(N^2/2) and par = 0 for odd N;
+    // so a wave's reads are consecutive words.  Zero guards of F (below R,
+    // above R), C (below and above K) entries take the out-of-cube
+    // neighbours, whose couplings are zero.
+    constexpr int HR = (NNN + 1) / 2, HB = NNN / 2, F = NN / 2, C = (NN + 1) / 2;
+    constexpr int YM = N / 2 + (N & 1), YP = N / 2;  // red y-, y+ (black: YP, YM)
+    constexpr int R0 = F, K0 = 2 * F + HR + C;        // split p: [0, NNN + 2 NN)
+    constexpr int ZR0 = F, ZK0 = F + HR + C;          // split wz: [0, NNN + NN)
+    static_assert(!RB || K0 + HB + C <= SH::kCells + 2 * SH::kGuard, "split p exceeds its array");
     static_assert(!RB || ZK0 + HB <= SH::kCells + SH::kGuard, "split wz exceeds its array");
     // A p stays in registers and the Jacobi scales s_c in the third LDS
     // array for the whole solve (s is read before and after each inner solve
@@ -229,6 +239,11 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     // in L2.
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
+    // row parity of the thread's pairs (even N; 0 for odd N), and slot q's cell
+    const int par = (N & 1) ? 0 : (((tid / (N / 2)) % N + tid / (N / 2) / N) & 1);
+    auto cell = [&](int q) __attribute__((always_inline)) {
+        return RB ? 2 * (tid + (q % CPH) * T) + (q >= CPH ? 1 - par : par) : tid + q * T;
+    };
     const float* E = a.E;
     const int code = a.code;
     float* P = sh.p + NN;  // valid for indices [-NN, NNN + NN)
@@ -241,8 +256,8 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     for (int g = tid; g < NN; g += T) { P[g - NN] = 0.f; P[NNN + g] = 0.f; WZ[g - NN] = 0.f; }
     if constexpr (RB) {
         // split wz guards: below ZR (black z- reads) and below ZK (red z- reads)
-        for (int g = tid; g < NNH; g += T) sh.wz[ZR0 - NNH + g] = 0.f;
-        for (int g = tid; g <= NNH; g += T) sh.wz[ZK0 - NNH - 1 + g] = 0.f;
+        for (int g = tid; g < F; g += T) sh.wz[ZR0 - F + g] = 0.f;
+        for (int g = tid; g < C; g += T) sh.wz[ZK0 - C + g] = 0.f;
     }
     if (tid == 0) {
         fdm_rhs<N>(E, code, sh.rhs_idx, sh.rhs_val);
@@ -257,7 +272,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     float sc[CPT];
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-        const int c = STEP * tid + off(q);
+        const int c = cell(q);
         sc[q] = 0.f;
         if (!edge(q) || c < NNN) {
             const int i = c % N, j = (c / N) % N, k = c / NN;
@@ -279,7 +294,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
     float w[4][CPT];  // x-, x+, y-, y+
 #pragma unroll
     for (int q = 0; q < CPT; q++) {
-        const int c = STEP * tid + off(q);
+        const int c = cell(q);
         if (!edge(q) || c < NNN) {
             const int i = c % N, j = (c / N) % N, k = c / NN;
             const float ec = EL[c], sc = P[c];
@@ -288,11 +303,25 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             w[2][q] = j > 0 ? harm(ec, EL[c - N]) * sc * P[c - N] : 0.f;
             w[3][q] = j < N - 1 ? harm(ec, EL[c + N]) * sc * P[c + N] : 0.f;
             const float wzc = k < N - 1 ? harm(ec, EL[c + NN]) * sc * P[c + NN] : 0.f;
-            if constexpr (RB) sh.wz[((c & 1) ? ZK0 : ZR0) + (c >> 1)] = wzc;
+            if constexpr (RB) sh.wz[(q >= CPH ? ZK0 : ZR0) + (c >> 1)] = wzc;
             else WZ[c] = wzc;
         } else {
 #pragma unroll
             for (int d = 0; d < 4; d++) w[d][q] = 0.f;
+        }
+    }
+    // Red-black: the red and the black cell of a pair are x neighbours, so
+    // one coupling serves both (the red cell's, which makes the scaled
+    // operator exactly symmetric in it); per pair the partner coupling wp,
+    // the red cell's other x coupling wor and the black cell's wob, and the
+    // y couplings in w[2], w[3]: one register per pair less than w[0..1].
+    float wp[RB ? CPH : 1], wor[RB ? CPH : 1], wob[RB ? CPH : 1];
+    if constexpr (RB) {
+#pragma unroll
+        for (int m = 0; m < CPH; m++) {
+            wp[m] = par ? w[0][m] : w[1][m];
+            wor[m] = par ? w[1][m] : w[0][m];
+            wob[m] = par ? w[0][m + CPH] : w[1][m + CPH];
         }
     }
     __syncthreads();
@@ -403,7 +432,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             float rloc = 0.f;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = STEP * tid + off(q);
+                const int c = cell(q);
                 xh[q] = 0.f;
                 rh[q] = (!edge(q) || c < NNN) ? R32[c] * SC[c] * inv : 0.f;
                 rloc += rh[q] * rh[q];
@@ -416,43 +445,48 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 // at K[(q - CPH)T].
                 lds_float* Rb = lds_base(sh.p + R0 + tid);
                 lds_float* Kb = lds_base(sh.p + K0 + tid);
+                // the other x neighbour of a red pair's cell is black pair
+                // h - 1 + 2 par, of a black pair's cell red pair h + 1 - 2 par
+                // (odd N: par = 0, fixed offsets from Kb, Rb)
+                lds_float* Kx = (N & 1) ? Kb - 1 : lds_base(sh.p + K0 + tid - 1 + 2 * par);
+                lds_float* Rx = (N & 1) ? Rb + 1 : lds_base(sh.p + R0 + tid + 1 - 2 * par);
                 lds_float* Zr = lds_base(sh.wz + ZR0 + tid);
                 lds_float* Zk = lds_base(sh.wz + ZK0 + tid);
                 // sum over the black neighbours of red slot q of w P
                 auto red_nb = [&](int q) __attribute__((always_inline)) {
                     const int j = q * T;
-                    float v = w[0][q] * Kb[j - 1];
-                    v += w[1][q] * Kb[j];
-                    v += w[2][q] * Kb[j - NH - 1];
-                    v += w[3][q] * Kb[j + NH];
-                    v += Zk[j - NNH - 1] * Kb[j - NNH - 1];
-                    v += Zr[j] * Kb[j + NNH];
+                    float v = wor[q] * Kx[j];
+                    v += wp[q] * Kb[j];
+                    v += w[2][q] * Kb[j - YM];
+                    v += w[3][q] * Kb[j + YP];
+                    v += Zk[j - C] * Kb[j - C];
+                    v += Zr[j] * Kb[j + F];
                     return v;
                 };
                 // sum over the red neighbours of black slot q of w P
                 auto black_nb = [&](int q) __attribute__((always_inline)) {
                     const int j = (q - CPH) * T;
-                    float v = w[0][q] * Rb[j];
-                    v += w[1][q] * Rb[j + 1];
-                    v += w[2][q] * Rb[j - NH];
-                    v += w[3][q] * Rb[j + NH + 1];
-                    v += Zr[j - NNH] * Rb[j - NNH];
-                    v += Zk[j] * Rb[j + NNH + 1];
+                    float v = wp[q - CPH] * Rb[j];
+                    v += wob[q - CPH] * Rx[j];
+                    v += w[2][q] * Rb[j - YP];
+                    v += w[3][q] * Rb[j + YM];
+                    v += Zr[j - F] * Rb[j - F];
+                    v += Zk[j] * Rb[j + C];
                     return v;
                 };
                 // guards of the split arrays (the p array held R32 until now)
-                for (int g = tid; g < NNH; g += T) {
-                    sh.p[R0 - NNH + g] = 0.f;
+                for (int g = tid; g < F; g += T) {
+                    sh.p[R0 - F + g] = 0.f;
                     sh.p[R0 + HR + g] = 0.f;
                 }
-                for (int g = tid; g <= NNH; g += T) {
-                    sh.p[K0 - NNH - 1 + g] = 0.f;
+                for (int g = tid; g < C; g += T) {
+                    sh.p[K0 - C + g] = 0.f;
                     sh.p[K0 + HB + g] = 0.f;
                 }
                 // b: red slots to R, black slots to K
 #pragma unroll
                 for (int q = 0; q < CPT; q++) {
-                    const int c = STEP * tid + off(q);
+                    const int c = cell(q);
                     if (!edge(q) || c < NNN) {
                         if (q < CPH) Rb[q * T] = rh[q];
                         else Kb[(q - CPH) * T] = rh[q];
@@ -464,7 +498,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 float rl = 0.f;
 #pragma unroll
                 for (int q = 0; q < CPH; q++) {
-                    const int c = STEP * tid + off(q);
+                    const int c = cell(q);
                     if (!edge(q) || c < NNN) {
                         rh[q] += red_nb(q);
                         rl += rh[q] * rh[q];
@@ -478,14 +512,14 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                     for (int it = 0; it < a.max_inner; it++) {
 #pragma unroll
                         for (int q = 0; q < CPH; q++) {
-                            const int c = STEP * tid + off(q);
+                            const int c = cell(q);
                             if (!edge(q) || c < NNN) Rb[q * T] = pv[q];
                         }
                         __syncthreads();
                         // black half sweep: t_b = B_br p
 #pragma unroll
                         for (int q = CPH; q < CPT; q++) {
-                            const int c = STEP * tid + off(q);
+                            const int c = cell(q);
                             if (!edge(q) || c < NNN) Kb[(q - CPH) * T] = black_nb(q);
                         }
                         __syncthreads();
@@ -494,7 +528,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                         float sp[CPH];
 #pragma unroll
                         for (int q = 0; q < CPH; q++) {
-                            const int c = STEP * tid + off(q);
+                            const int c = cell(q);
                             sp[q] = 0.f;
                             if (!edge(q) || c < NNN) {
                                 sp[q] = pv[q] - red_nb(q);
@@ -506,7 +540,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                         float rl2 = 0.f;
 #pragma unroll
                         for (int q = 0; q < CPH; q++) {
-                            const int c = STEP * tid + off(q);
+                            const int c = cell(q);
                             if (!edge(q) || c < NNN) {
                                 xh[q] += alpha * pv[q];
                                 rh[q] -= alpha * sp[q];
@@ -526,13 +560,13 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 // or the one before the loop, ordered every earlier read of R)
 #pragma unroll
                 for (int q = 0; q < CPH; q++) {
-                    const int c = STEP * tid + off(q);
+                    const int c = cell(q);
                     if (!edge(q) || c < NNN) Rb[q * T] = xh[q];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int q = CPH; q < CPT; q++) {
-                    const int c = STEP * tid + off(q);
+                    const int c = cell(q);
                     if (!edge(q) || c < NNN) xh[q] = rh[q] + black_nb(q);
                 }
                 __syncthreads();  // every read of the split arrays done before P is rewritten
@@ -543,7 +577,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                 lds_float* Zb = lds_base(sh.wz + tid);
 #pragma unroll
                 for (int q = 0; q < CPT; q++) {
-                    const int c = STEP * tid + off(q);
+                    const int c = cell(q);
                     if (!edge(q) || c < NNN) P[c] = rh[q];
                 }
                 __syncthreads();
@@ -552,7 +586,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                     float apv[CPT];
 #pragma unroll
                     for (int q = 0; q < CPT; q++) {
-                        const int c = STEP * tid + off(q);
+                        const int c = cell(q);
                         apv[q] = 0.f;
                         if (!edge(q) || c < NNN) {
                             const int o = off(q) + NN;  // P index of cell c relative to Pb
@@ -573,7 +607,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                     float rl = 0.f;
 #pragma unroll
                     for (int q = 0; q < CPT; q++) {
-                        const int c = STEP * tid + off(q);
+                        const int c = cell(q);
                         if (!edge(q) || c < NNN) {
                             xh[q] += alpha * Pb[off(q) + NN];
                             rh[q] -= alpha * apv[q];
@@ -587,7 +621,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
                     rr = rr_new;
 #pragma unroll
                     for (int q = 0; q < CPT; q++) {
-                        const int c = STEP * tid + off(q);
+                        const int c = cell(q);
                         if (!edge(q) || c < NNN) Pb[off(q) + NN] = rh[q] + beta * Pb[off(q) + NN];
                     }
                     __syncthreads();
@@ -602,7 +636,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             yscale = rnorm;
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = STEP * tid + off(q);
+                const int c = cell(q);
                 if (!edge(q) || c < NNN) XV[c] = xh[q];
             }
         } else {
@@ -610,7 +644,7 @@ __device__ void fdm_cube(const CubeJob& a, SH& sh)
             asm volatile("" : "+v"(Yv));
 #pragma unroll
             for (int q = 0; q < CPT; q++) {
-                const int c = STEP * tid + off(q);
+                const int c = cell(q);
                 if (!edge(q) || c < NNN) Yv[c] += (double)SC[c] * (double)xh[q] * rnorm;
             }
         }
