@@ -102,6 +102,53 @@ __device__ int wave_cdf_draw(Ptr w, int M, uint32_t u)
     return M - 1;
 }
 
+// wave_cdf_draw over a global array of at most 64 * MPL doubles, read from
+// HBM once: each lane keeps its bins i = lane + 64 k in registers for the
+// max, total and scan passes (the passes and their summation orders are
+// wave_cdf_draw's, so the drawn bin is the same).  `absum` receives the
+// lane-strided sum of |w_i| reduced over the wave (gradient queries' W), in
+// the order of a loop over i = lane, lane + 64, ...
+template <int MPL>
+__device__ int wave_cdf_draw_regs(const double* __restrict__ w, int M, uint32_t u, double& absum)
+{
+    const int lane = threadIdx.x & 63;
+    double g[MPL];
+    double m = 0.0, sa = 0.0;
+#pragma unroll
+    for (int k = 0; k < MPL; k++) {
+        const int i = lane + 64 * k;
+        g[k] = i < M ? w[i] : 0.0;
+        const double a = g[k] < 0 ? -g[k] : g[k];
+        m = a > m ? a : m;
+        if (i < M) sa += fabs(g[k]);
+    }
+    absum = wave_sum(sa);
+    m = wave_max(m);
+    if (!(m > 0) || isinf(m)) return -1;
+    const int s = 24 - frexp_exp(m);
+    uint32_t qv[MPL];
+    uint64_t total = 0;
+#pragma unroll
+    for (int k = 0; k < MPL; k++) {
+        qv[k] = lane + 64 * k < M ? quant(g[k], s) : 0u;
+        total += qv[k];
+    }
+    total = wave_sum_u64(total);
+    const uint64_t lo = (uint64_t)u * (total & 0xFFFFFFFFull);
+    const uint64_t hi = (uint64_t)u * (total >> 32);
+    const uint64_t r = hi + (lo >> 32);  // floor(u * total / 2^32), total < 2^48
+    uint64_t base = 0;
+#pragma unroll
+    for (int k = 0; k < MPL; k++) {
+        if (64 * k >= M) break;  // uniform
+        const uint32_t incl = wave_inclusive_scan(qv[k]);
+        const uint64_t hit = __ballot(base + incl > r);
+        if (hit) return 64 * k + __ffsll((unsigned long long)hit) - 1;
+        base += (uint32_t)__builtin_amdgcn_readlane((int)incl, 63);
+    }
+    return M - 1;
+}
+
 // Panel centre of tile t on face f (locate_index_simple_kernel,
 // inference_cpp/src/cuda_kernels.cu:219-264), for any N.
 __device__ __forceinline__ void panel_point(int N, int t, int f, float* o)
@@ -134,6 +181,9 @@ __host__ __device__ __forceinline__ int axis_face_remap(int f, int axis)
 //   Poisson  (!grad): o = (x, y, z)
 //   gradient (grad):  o = (x, y, z, W sign(G_i), eps_centre), W = sum |G|
 // A failed query (bad, or no drawable bin) gets NaN in every output.
+// REGS: G is read once into registers (wave_cdf_draw_regs, for kernels with
+// registers to spare); otherwise every pass reads it from memory.
+template <bool REGS = false>
 __device__ inline void flat_draw_wave(const double* G, int N, bool grad, float center_eps, uint64_t seed,
                                       uint64_t counter, bool bad, float* o)
 {
@@ -142,7 +192,10 @@ __device__ inline void flat_draw_wave(const double* G, int N, bool grad, float c
     const int width = grad ? 5 : 3;
     uint32_t u[4];
     query_uniforms(seed, counter, 0u, u);
-    const int i = bad ? -1 : wave_cdf_draw<double>(G, M, u[0]);
+    double wsum_regs = 0.0;
+    int i;
+    if constexpr (REGS) i = bad ? -1 : wave_cdf_draw_regs<(6 * kMaxN * kMaxN + 63) / 64>(G, M, u[0], wsum_regs);
+    else i = bad ? -1 : wave_cdf_draw<double>(G, M, u[0]);
     if (i < 0) {
         if (lane < width) o[lane] = __builtin_nanf("");
         return;
@@ -156,9 +209,11 @@ __device__ inline void flat_draw_wave(const double* G, int N, bool grad, float c
         }
         return;
     }
-    double wsum = 0.0;
-    for (int k = lane; k < M; k += 64) wsum += fabs(G[k]);
-    wsum = wave_sum(wsum);
+    double wsum = wsum_regs;
+    if constexpr (!REGS) {
+        for (int k = lane; k < M; k += 64) wsum += fabs(G[k]);
+        wsum = wave_sum(wsum);
+    }
     if (lane == 0) {
         float p[3];
         panel_point(N, t, f, p);

@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void sample_flat_kernel(FlatSampleArgs a)
     if (b >= a.B) return;
     const uint64_t base = a.ctr_dev ? *a.ctr_dev : a.counter;
     const bool grad = a.axis != nullptr;
-    flat_draw_wave(a.G + a.G_off[b], a.N[b], grad, grad && a.center_eps ? a.center_eps[b] : 0.0f, a.seed,
+    flat_draw_wave<true>(a.G + a.G_off[b], a.N[b], grad, grad && a.center_eps ? a.center_eps[b] : 0.0f, a.seed,
                    a.qid ? a.qid[b] : base + (uint64_t)b, a.bad && a.bad[b], a.out + (size_t)b * (grad ? 5 : 3));
 }
 
