@@ -449,14 +449,17 @@ __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetD
     dw_conv<NTH, 24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
     pw_mfma<NTH, 32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.frag_lo[3], w.pw_bias[3], s.gtab); __syncthreads();
     dw_conv<NTH, 24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 32, 16, 16, 32, 32>(s.X, s.X, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
-    dw_conv<NTH, 16, 2, 32, 32>(s.X, s.Y, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 16, 16, 16, 32, 32>(s.Y, s.Y, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
-    dw_conv<NTH, 16, 2, 32, 32>(s.Y, s.X, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 16, 8, 16, 32, 32>(s.X, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
+    // from 16 channels on the activations use a 16-channel pixel stride (the
+    // first half of each buffer): the depthwise reads of consecutive pixels
+    // then fall on distinct banks, where the 32-channel stride aliases them
+    pw_mfma<NTH, 32, 16, 16, 32, 16>(s.X, s.Y, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 2, 16, 16>(s.Y, s.X, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 16, 16, 16, 16>(s.X, s.X, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 2, 16, 16>(s.X, s.Y, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 8, 16, 16, 16>(s.Y, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
-    dw_conv<NTH, 8, 3, 32, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
+    dw_conv<NTH, 8, 3, 16, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
     pw_valu<NTH, 8, 4, 8, 4>(fY, fX, w.pw_w[7], w.pw_bias[7], s.gtab); __syncthreads();
     head_conv<NTH, 4, 4>(fX, fY, w.head_w, w.head_b);
 }
