@@ -128,10 +128,13 @@ __device__ __forceinline__ void store_pair(float* d, float a, float b) { *(float
 //    nine taps sit at fixed offsets from the pixel, so an item costs its
 //    index, nine loads, the products and the GELUs; at dilation 1 an item is
 //    three adjacent pixels of a row (the 21 interior columns make 7 groups);
-//  * border pixels: a tap outside the image reads `zero` (a zeroed LDS slot)
-//    instead, chosen per tap by a select, so the nine loads of an item still
-//    issue back to back.
-// Both passes sum the taps in the same order.  A thread's channel pair, and
+//  * border pixels: at dilation 1 the edges are taken three pixels per item
+//    as well (rows along x, columns along y, each reading only its two
+//    in-image window lines), the corners one by one; at dilation 2 and 3,
+//    and at the corners, a tap outside the image reads `zero` (a zeroed LDS
+//    slot) instead, chosen per tap by a select, so the nine loads of an item
+//    still issue back to back.
+// All paths sum the taps in the same order.  A thread's channel pair, and
 // so its 18 weights, stays fixed: items advance by the largest multiple of
 // C/2 up to 256 (252 threads at 24 channels, the other 4 idle here).
 template <int NTH, int C, int DIL, int CSI, int CSO, typename TI, typename TO>
@@ -203,6 +206,108 @@ __device__ void dw_conv_direct(const TI* __restrict__ in, TO* __restrict__ out, 
             }
             finish(p, v0, v1);
         }
+    }
+    if constexpr (DIL == 1 && NI % 3 == 0) {
+        // Border at dilation 1, three pixels per item like the interior: the
+        // top and bottom rows (x = 1..21) in groups along x with their two
+        // in-image window rows (2 x 5 loads), the left and right columns
+        // (y = 1..21) in groups along y with their two in-image window
+        // columns (5 x 2 loads), then the four corners one by one.  The taps
+        // outside the image are left out of the sums (the general border
+        // path adds them as 0 x w); the others are summed in the same
+        // (ky, kx) order.
+        constexpr int G = 3, NG = NI / G;
+        for (int item = threadIdx.x; item < CP * (4 * NG + 4); item += STRIDE) {
+            const int q = item / CP;
+            if (q < 2 * NG) {
+                const bool bot = q >= NG;
+                const int x0 = 1 + G * (q - (bot ? NG : 0));
+                const int rA = bot ? kN - 2 : 0;  // upper in-image window row
+                const TI* c = in + (rA * kN + x0 - 1) * CSI + 2 * cp;
+                float a0[G + 2], a1[G + 2], b0v[G + 2], b1v[G + 2];
+#pragma unroll
+                for (int kx = 0; kx < G + 2; kx++) {
+                    load_pair(c + kx * CSI, a0[kx], a1[kx]);
+                    load_pair(c + (kN + kx) * CSI, b0v[kx], b1v[kx]);
+                }
+                // window row A is tap row 1 (top) or 0 (bottom), row B the next
+                float wa0[3], wa1[3], wb0[3], wb1[3];
+#pragma unroll
+                for (int kx = 0; kx < 3; kx++) {
+                    wa0[kx] = bot ? w0[kx] : w0[3 + kx];
+                    wa1[kx] = bot ? w1[kx] : w1[3 + kx];
+                    wb0[kx] = bot ? w0[3 + kx] : w0[6 + kx];
+                    wb1[kx] = bot ? w1[3 + kx] : w1[6 + kx];
+                }
+                const int py = bot ? kN - 1 : 0;
+#pragma unroll
+                for (int j = 0; j < G; j++) {
+                    float s0 = b0, s1 = b1;
+#pragma unroll
+                    for (int kx = 0; kx < 3; kx++) {
+                        s0 += wa0[kx] * a0[j + kx];
+                        s1 += wa1[kx] * a1[j + kx];
+                    }
+#pragma unroll
+                    for (int kx = 0; kx < 3; kx++) {
+                        s0 += wb0[kx] * b0v[j + kx];
+                        s1 += wb1[kx] * b1v[j + kx];
+                    }
+                    store_pair(out + (py * kN + x0 + j) * CSO + 2 * cp, gelu_tab(s0, gtab), gelu_tab(s1, gtab));
+                }
+            } else if (q < 4 * NG) {
+                const bool right = q >= 3 * NG;
+                const int y0 = 1 + G * (q - (right ? 3 * NG : 2 * NG));
+                const int cA = right ? kN - 2 : 0;  // left in-image window column
+                const TI* c = in + ((y0 - 1) * kN + cA) * CSI + 2 * cp;
+                float a0[G + 2], a1[G + 2], b0v[G + 2], b1v[G + 2];
+#pragma unroll
+                for (int ky = 0; ky < G + 2; ky++) {
+                    load_pair(c + ky * kN * CSI, a0[ky], a1[ky]);
+                    load_pair(c + (ky * kN + 1) * CSI, b0v[ky], b1v[ky]);
+                }
+                // window column A is tap column 1 (left) or 0 (right), B the next
+                float wa0[3], wa1[3], wb0[3], wb1[3];
+#pragma unroll
+                for (int ky = 0; ky < 3; ky++) {
+                    wa0[ky] = right ? w0[3 * ky] : w0[3 * ky + 1];
+                    wa1[ky] = right ? w1[3 * ky] : w1[3 * ky + 1];
+                    wb0[ky] = right ? w0[3 * ky + 1] : w0[3 * ky + 2];
+                    wb1[ky] = right ? w1[3 * ky + 1] : w1[3 * ky + 2];
+                }
+                const int px = right ? kN - 1 : 0;
+#pragma unroll
+                for (int j = 0; j < G; j++) {
+                    float s0 = b0, s1 = b1;
+#pragma unroll
+                    for (int ky = 0; ky < 3; ky++) {
+                        s0 += wa0[ky] * a0[j + ky];
+                        s1 += wa1[ky] * a1[j + ky];
+                        s0 += wb0[ky] * b0v[j + ky];
+                        s1 += wb1[ky] * b1v[j + ky];
+                    }
+                    store_pair(out + ((y0 + j) * kN + px) * CSO + 2 * cp, gelu_tab(s0, gtab), gelu_tab(s1, gtab));
+                }
+            } else {
+                const int k = q - 4 * NG;
+                const int y = (k >> 1) ? kN - 1 : 0, x = (k & 1) ? kN - 1 : 0;
+                const int p = y * kN + x;
+                const TI* c = in + p * CSI + 2 * cp;
+                const bool rv[3] = {y >= 1, true, y + 1 < kN};
+                const bool cv[3] = {x >= 1, true, x + 1 < kN};
+                float v0[9], v1[9];
+#pragma unroll
+                for (int ky = 0; ky < 3; ky++) {
+#pragma unroll
+                    for (int kx = 0; kx < 3; kx++) {
+                        const TI* src = (rv[ky] && cv[kx]) ? c + ((ky - 1) * kN + (kx - 1)) * CSI : zero;
+                        load_pair(src, v0[3 * ky + kx], v1[3 * ky + kx]);
+                    }
+                }
+                finish(p, v0, v1);
+            }
+        }
+        return;
     }
     for (int item = threadIdx.x; item < CP * NB; item += STRIDE) {
         const int q = item / CP;
