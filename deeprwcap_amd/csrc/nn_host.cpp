@@ -302,9 +302,17 @@ int build_sel(const TensorMap& m, bool weighted, Arena& ar, SelNetDev& d)
             for (int q = 0; q < K; q++) W[(size_t)o * K + q] *= sc[o];
             b[o] = sc[o] * bb->data[o] + sh[o];
         }
-        ar.add(pack_frag(W, co, K), (const half_t**)&d.frag[l]);
+        if (l == 0) {
+            ar.add(conv1_pairs(W, co), &d.conv1_w);
+        } else {
+            // conv3d_mfma's K order: k = tap * ci + c (channel innermost)
+            std::vector<float> Wt(W.size());
+            for (int o = 0; o < co; o++)
+                for (int c = 0; c < ci; c++)
+                    for (int q = 0; q < 27; q++) Wt[(size_t)o * K + q * ci + c] = W[(size_t)o * K + c * 27 + q];
+            ar.add(pack_frag(Wt, co, K), (const half_t**)&d.frag[l]);
+        }
         ar.add(pad(b, 64), &d.bias[l]);
-        if (l == 0) ar.add(conv1_pairs(W, co), &d.conv1_w);
     }
     ar.add(gelu_table(kSelGeluTab, kSelGeluLo, kSelGeluInvH), &d.gtab);
     const std::vector<int> dims = weighted ? std::vector<int>{512, 256, 128, 7} : std::vector<int>{128, 64, 6};

@@ -13,16 +13,19 @@
 
 namespace drw {
 
-// Conv3d(CIN -> COUT, k3, s2, no padding) over in[ci][z][y][x] (fp16,
-// strides SC, SZ, SY, 1) as an implicit GEMM (rows = output positions,
-// k = ci*27 + tap) with bias + GELU, into out[co][oz*DOUT^2+oy*DOUT+ox],
-// fp16 or fp32 (FP32_OUT); `out` must not overlap `in`.  koff (LDS,
-// >= KT*16 ints) receives the im2col offset of every k (-1 past K), so a
-// gathered element costs one add and one LDS read.  With COUT < 16 only COUT
-// of the 16 MFMA columns hold outputs; the valid ones are first moved across
-// lanes (ds_bpermute) so that every lane evaluates 16*COUT/64 GELUs instead
-// of 4.  Ends with a barrier: outputs visible, koff free for the next layer.
-template <int CIN, int COUT, int DIN, int DOUT, int SC, int SZ, int SY, bool FP32_OUT, int NTH = 256>
+// Conv3d(CIN -> COUT, k3, s2, no padding) over in[z][y][x][ci] (fp16,
+// channel innermost, DIN^3 positions) as an implicit GEMM (rows = output
+// positions, k = tap*CIN + ci, tap = 9 tz + 3 ty + tx) with bias + GELU,
+// into out[oz][oy][ox][co] (fp16, channel innermost, for the next layer) or
+// out[co][oz*DOUT^2+oy*DOUT+ox] (fp32, FP32_OUT: the flatten order of the
+// Linear head); `out` must not overlap `in`.  A lane's four K slots of an
+// MFMA step are four consecutive channels of one tap, so its operand is one
+// 8-byte LDS read at the tap's offset (koff, LDS, 28 ints: -1 past the last
+// tap).  With COUT < 16 only COUT of the 16 MFMA columns hold outputs; the
+// valid ones are first moved across lanes (ds_bpermute) so that every lane
+// evaluates 16*COUT/64 GELUs instead of 4.  Ends with a barrier: outputs
+// visible, koff free for the next layer.
+template <int CIN, int COUT, int DIN, int DOUT, bool FP32_OUT, int NTH = 256>
 __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, const float* bias, int* koff)
 {
     constexpr int DO2 = DOUT * DOUT, M = DOUT * DO2;
@@ -32,14 +35,10 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
     constexpr int NW = NTH / 64;
     constexpr int TP = (M + 15) / 16, TPW = (TP + NW - 1) / NW;  // tiles / tiles per wave
     static_assert(COUT >= 16 || (16 * COUT) % 64 == 0, "compacted tiles need COUT a multiple of 4");
-    for (int k = threadIdx.x; k < KT * 16; k += NTH) {
-        int o = -1;
-        if (k < K) {
-            const int ci = k / 27, tap = k - 27 * ci;
-            o = ci * SC + (tap / 9) * SZ + ((tap / 3) % 3) * SY + tap % 3;
-        }
-        koff[k] = o;
-    }
+    static_assert(CIN % 4 == 0, "a lane's four K slots are channels of one tap");
+    static_assert(KT * 16 <= 28 * CIN, "koff covers every tap a step can reach");
+    for (int t = threadIdx.x; t < 28; t += NTH)
+        koff[t] = t < 27 ? ((t / 9) * DIN * DIN + ((t / 3) % 3) * DIN + t % 3) * CIN : -1;
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, kq = 4 * (lane >> 4);
@@ -50,6 +49,26 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 #pragma unroll
             for (int kt = 0; kt < KT; kt++) bw[n][kt] = frag[(n * KT + kt) * 64 + lane];
     }
+    // operand of output row m (base: its window's first element, -1 past M)
+    // for step kt: channels c0..c0+3 of tap (kt*16 + kq) / CIN
+    auto gather = [&](int base, int kt) __attribute__((always_inline)) {
+        const int k0 = kt * 16 + kq, tap = k0 / CIN, c0 = k0 - tap * CIN;
+        const int o = koff[tap];
+        half4_t a = {(half_t)0.f, (half_t)0.f, (half_t)0.f, (half_t)0.f};
+        if (base >= 0 && o >= 0) a = *(const half4_t*)(in + base + o + c0);
+        return a;
+    };
+    auto row_base = [](int m) __attribute__((always_inline)) {
+        if (m >= M) return -1;
+        const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
+        return (2 * oz * DIN * DIN + 2 * oy * DIN + 2 * ox) * CIN;
+    };
+    auto store = [&](int ch, int mm, float g) __attribute__((always_inline)) {
+        if (mm < M) {
+            if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
+            else ((half_t*)outp)[mm * COUT + ch] = (half_t)g;
+        }
+    };
     if constexpr (NT > 1 && TP < NW) {
         // fewer row tiles than waves (FaceSelectorWeight's last layer: one
         // tile of 8 positions, 64 channels): one (row tile, channel tile)
@@ -61,31 +80,12 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
             const int item = wave + NW * u;
             if (item >= ITEMS) continue;
             const int t = item / NT, n = item - t * NT;
-            const int m = t * 16 + r;
-            int base = -1;
-            if (m < M) {
-                const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
-                base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
-            }
+            const int base = row_base(t * 16 + r);
             float4_t acc = {0.f, 0.f, 0.f, 0.f};
-            for (int kt = 0; kt < KT; kt++) {
-                const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
-                const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
-                half4_t a;
-#pragma unroll
-                for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
-                acc = mfma16(a, frag[(n * KT + kt) * 64 + lane], acc);
-            }
+            for (int kt = 0; kt < KT; kt++) acc = mfma16(gather(base, kt), frag[(n * KT + kt) * 64 + lane], acc);
             const int ch = n * 16 + r;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int mm = t * 16 + kq + i;
-                const float g = gelu(acc[i] + bias[ch]);
-                if (mm < M) {
-                    if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
-                    else ((half_t*)outp)[ch * M + mm] = (half_t)g;
-                }
-            }
+            for (int i = 0; i < 4; i++) store(ch, t * 16 + kq + i, gelu(acc[i] + bias[ch]));
         }
         __syncthreads();
         return;
@@ -94,21 +94,12 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
     for (int u = 0; u < TPW; u++) {
         const int t = wave + NW * u;
         if (t >= TP) continue;
-        const int m = t * 16 + r;
-        int base = -1;
-        if (m < M) {
-            const int oz = m / DO2, oy = (m / DOUT) % DOUT, ox = m % DOUT;
-            base = 2 * oz * SZ + 2 * oy * SY + 2 * ox;
-        }
+        const int base = row_base(t * 16 + r);
         float4_t acc[NT];
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[n] = (float4_t){0.f, 0.f, 0.f, 0.f};
         for (int kt = 0; kt < KT; kt++) {
-            const int4 o4 = *(const int4*)(koff + kt * 16 + kq);
-            const int oo[4] = {o4.x, o4.y, o4.z, o4.w};
-            half4_t a;
-#pragma unroll
-            for (int j = 0; j < 4; j++) a[j] = (base >= 0 && oo[j] >= 0) ? in[base + oo[j]] : (half_t)0.f;
+            const half4_t a = gather(base, kt);
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 half4_t b;
@@ -139,11 +130,7 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
                 mm = t * 16 + kq + (j & 3);
                 v = acc[j >> 2][j & 3];
             }
-            const float g = gelu(v + bias[ch]);
-            if (mm < M) {
-                if (FP32_OUT) ((float*)outp)[ch * M + mm] = g;
-                else ((half_t*)outp)[ch * M + mm] = (half_t)g;
-            }
+            store(ch, mm, gelu(v + bias[ch]));
         }
     }
     __syncthreads();
@@ -153,7 +140,7 @@ __device__ void conv3d_mfma(const half_t* in, void* outp, const half4_t* frag, c
 // one output z-slab per phase, in place: slab oz reads input planes
 // 2oz..2oz+2, so after the phase's barrier the planes below 2oz+2 are dead
 // and the slab's outputs (held in registers until then) are written over
-// them as out[oz][c][oy*11+ox] (C1 * 121 <= 2 * 529).  An item is one output
+// them as out[oz][oy*11+ox][c] (channel innermost, C1 * 121 <= 2 * 529).  An item is one output
 // pixel of the slab and one channel pair; its 27 taps lie inside the cube
 // for every output, so they are LDS reads at fixed offsets from one base
 // address.  The slab base offset is even and the plane and row strides
@@ -216,8 +203,10 @@ __device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, c
 #pragma unroll
         for (int u = 0; u < IPT; u++) {
             if (pix[u] < DO2) {
-                o[(2 * cp) * DO2 + pix[u]] = (half_t)held[u][0];
-                o[(2 * cp + 1) * DO2 + pix[u]] = (half_t)held[u][1];
+                half2_t h;
+                h[0] = (half_t)held[u][0];
+                h[1] = (half_t)held[u][1];
+                *(half2_t*)(o + pix[u] * C1 + 2 * cp) = h;
             }
         }
     }
@@ -225,15 +214,15 @@ __device__ void conv1_direct(const half_t* cube, half_t* out, const half_t* w, c
 }
 
 // LDS of one query: the cube, the first conv's output (written slab by slab
-// over the cube planes it no longer needs, slab-major [oz][c][11*11]) and
-// the second conv's output ([c][125], after the first's) share one region:
+// over the cube planes it no longer needs, [oz][11*11][c]) and the second
+// conv's output ([125][c], after the first's) share one region:
 // 26.4 KB in all for FaceSelector (6 workgroups per CU), 27.4 KB for
 // FaceSelectorWeight (5 per CU).
 template <int C1, int C2>
 struct SelSmem {
     static constexpr int kRegion = kNNN > C1 * 1331 + C2 * 125 ? kNNN : C1 * 1331 + C2 * 125;
-    alignas(16) int koff[27 * 16];  //  1,728 B  im2col offsets of the current layer (read as int4);
-                                    //           the first conv's GELU table before that
+    alignas(16) int koff[27 * 16];  //  1,728 B  tap offsets of the current layer (28 ints); the
+                                    //           first conv's GELU table before that
     half_t cube[kRegion];           // >= 24,334 B
     VoxDesc desc;
 };
@@ -280,11 +269,11 @@ __device__ __forceinline__ void selector_backbone_body(const VoxDesc& d, half_t*
         }
     }
     __syncthreads();
-    half_t* l1 = cube;                 // [11][C1][121], over the cube
-    half_t* l2 = cube + C1 * 1331;     // [C2][125], after l1
+    half_t* l1 = cube;                 // [11][121][C1], over the cube
+    half_t* l2 = cube + C1 * 1331;     // [125][C2], after l1
     conv1_direct<C1, NTH>(cube, l1, w.conv1_w, w.bias[0], (const float2*)koff);
-    conv3d_mfma<C1, C2, 11, 5, 121, C1 * 121, 11, false, NTH>(l1, l2, w.frag[1], w.bias[1], koff);
-    conv3d_mfma<C2, C3, 5, 2, 125, 25, 5, true, NTH>(l2, feat, w.frag[2], w.bias[2], koff);
+    conv3d_mfma<C1, C2, 11, 5, false, NTH>(l1, l2, w.frag[1], w.bias[1], koff);
+    conv3d_mfma<C2, C3, 5, 2, true, NTH>(l2, feat, w.frag[2], w.bias[2], koff);
 }
 
 // Linear layer on MFMA for 16 rows: X[16][K] (fp16, LDS) -> Y[16][N].
