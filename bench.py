@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 PRED_FLOP = 2 * 646_438      # PoissonPredictor MACs per face evaluation (architecture.py:44-72)
 SEL_FLOP = 2 * 287_972       # PoissonSelector MACs per query, convolutions and Linear head (architecture.py:176-205)
-SEL_FLOP = 2 * 287_972       # PoissonSelector MACs per query (architecture.py:176-205)
+HEAD_FLOP = 2 * (128 * 64 + 64 * 6)  # its Linear head alone (128 -> 64 -> 6)
 HBM_PEAK_GBS = 8000.0
 FP16_PEAK_TFLOPS = 2500.0
 FP32_VALU_PEAK_TFLOPS = 157.3
@@ -564,8 +564,8 @@ def main():
         log(f"NN: {value:,.0f} queries/s, {1000 * dt / args.steps:.3f} ms/step; fused query kernel {fused_ms:.3f} ms "
             f"per launch")
     else:
-        log(f"NN: {value:,.0f} queries/s, {1000 * dt / args.steps:.3f} ms/step; face kernel {face_ms:.3f} ms, "
-            f"selector {sel_ms:.3f} ms per launch")
+        log(f"NN: {value:,.0f} queries/s, {1000 * dt / args.steps:.3f} ms/step; head + predictor kernel {face_ms:.3f} ms, "
+            f"selector backbone {sel_ms:.3f} ms per launch")
 
     # FDM path on the same batch (the batched FP32-FDM query rate)
     fdm = DNNSolverPoisson(0)
@@ -607,13 +607,15 @@ def main():
 
     shape = _lib.lib().drw_get_launch_shape  # the template arguments rocprofv3 names the kernel with
     if fused:
-        dom_kernel, dom_flop, dom_ms = f"poisson_query_kernel<{shape(3, B)}>", SEL_FLOP + PRED_FLOP, fused_ms
+        dom_kernel, dom_flop, dom_ms = f"poisson_query_kernel<{shape(3, B)}, false>", SEL_FLOP + PRED_FLOP, fused_ms
         dom_note = (f"algorithmic {SEL_FLOP + PRED_FLOP:,} FLOP per query (PoissonSelector incl. its Linear head "
                     f"{SEL_FLOP:,} + PoissonPredictor {PRED_FLOP:,}); the kernel is bound by VALU issue "
                     "(depthwise conv + GELU + voxel lookups), see profiles/pmc_poisson_n23_b8192.md")
     else:
-        dom_kernel, dom_flop, dom_ms = f"face_query_kernel<0, {shape(0, B)}>", PRED_FLOP, face_ms
-        dom_note = (f"algorithmic {PRED_FLOP:,} FLOP per query; the kernel is bound by VALU issue "
+        # the selector backbone kernel, then head + face draw + predictor in one
+        dom_kernel, dom_flop, dom_ms = f"poisson_query_kernel<{shape(0, B)}, true>", HEAD_FLOP + PRED_FLOP, face_ms
+        dom_note = (f"algorithmic {HEAD_FLOP + PRED_FLOP:,} FLOP per query (PoissonSelector's Linear head "
+                    f"{HEAD_FLOP:,} + PoissonPredictor {PRED_FLOP:,}); the kernel is bound by VALU issue "
                     "(depthwise conv + GELU), see profiles/pmc_poisson_n23_b8192.md")
     achieved = B * dom_flop / (dom_ms * 1e-3) / 1e12
     traffic, valu_busy = face_counters(B, dom_kernel)
@@ -648,8 +650,7 @@ def main():
                      "3 more mantissa bits than the bf16 the metric names",
         "latency_ms_batch64": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "stage_ms_per_launch": ({"poisson_query_fused": fused_ms} if fused else
-                                {"selector_backbone": ms[0] / max(cnt[0], 1), "selector_head": ms[1] / max(cnt[1], 1),
-                                 "face_predictor": face_ms}),
+                                {"selector_backbone": ms[0] / max(cnt[0], 1), "head_and_predictor": face_ms}),
         "roofline": {"kernel": dom_kernel, "bound": "mfma", "achieved": achieved,
                      "peak": FP16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP16_PEAK_TFLOPS,
                      "traffic": traffic, "valu_issue_frac": valu_busy, "note": dom_note},

@@ -44,6 +44,8 @@ struct FaceQueryArgs {
     uint64_t* fb_stats;
     float* probs_out;        // fused Poisson query: [B, 6] face probabilities (optional)
     int32_t* face_out;       // fused Poisson query: [B] drawn face (optional)
+    const float* feat_in;    // Poisson query: [B, 128] PoissonSelector features from the backbone
+                             // kernel; the query kernel then runs the head and the predictor only
 };
 
 // mode 0: PoissonPredictor, 1: Gradient1Predictor, 2: Gradient2Predictor.
@@ -53,6 +55,8 @@ struct SelNetDev;
 // The whole Poisson query in one kernel (sample mode): one workgroup per
 // query runs the PoissonSelector, its Linear head, the face draw and the
 // PoissonPredictor on that face, with threads = 256 or 512 per workgroup.
+// With a.feat_in the selector backbone has already run (its kernel wrote
+// the features) and the workgroup starts at the head.
 int poisson_query_launch(const FaceQueryArgs& a, const SelNetDev& sel, const FaceNetDev& pred, int threads,
                          hipStream_t stream);
 

@@ -119,20 +119,29 @@ static_assert(sizeof(PoissonSmem) == sizeof(FaceSmem<16>), "the fused kernel's L
 // head (the batched kernels' device code, so every value is the separate
 // kernels'); softmax and face draw (word 0); PoissonPredictor on the face;
 // output head, tile draw (word 1), locate (dnn_solver_poisson.cpp:86-111).
-template <int NTH>
+// FEAT_IN: the features come from the selector backbone kernel (a.feat_in),
+// so the workgroup starts at the head: for large batches the backbone runs
+// as its own kernel (26 KB of LDS, six workgroups per CU) and the head, the
+// face draw and the predictor share one launch, without the head kernel's
+// launch and its wait.
+template <int NTH, bool FEAT_IN>
 __global__ __launch_bounds__(NTH) void poisson_query_kernel(FaceQueryArgs a, SelNetDev sw, FaceNetDev pw)
 {
     __shared__ PoissonSmem s;
     FaceSmem<16>& f = s.f;
     const int b = blockIdx.x;
     if (threadIdx.x < 64) vox_desc_build(f.desc, a.layers, a.cuboids, b);
-    else
+    else if (!FEAT_IN)
         for (int i = threadIdx.x - 64; i < kSelGeluTab; i += NTH - 64) ((float2*)s.sel.koff)[i] = sw.gtab[i];
     for (int i = threadIdx.x; i < kGeluTabEntries; i += NTH) f.gtab[i] = pw.gtab[i];
     if (threadIdx.x < 4) f.zero[threadIdx.x] = 0.f;
-    __syncthreads();
-    selector_backbone_body<false, NTH>(f.desc, s.sel.cube, s.sel.koff, false, 2, sw, s.sel.feat);
-    for (int i = threadIdx.x; i < 128; i += NTH) s.head.x[i] = (half_t)s.sel.feat[i];
+    if constexpr (FEAT_IN) {
+        for (int i = threadIdx.x; i < 128; i += NTH) s.head.x[i] = (half_t)a.feat_in[(size_t)b * 128 + i];
+    } else {
+        __syncthreads();
+        selector_backbone_body<false, NTH>(f.desc, s.sel.cube, s.sel.koff, false, 2, sw, s.sel.feat);
+        for (int i = threadIdx.x; i < 128; i += NTH) s.head.x[i] = (half_t)s.sel.feat[i];
+    }
     __syncthreads();
     linear_row<128, 64, true, NTH>(s.head.x, s.head.y1, nullptr, sw.frag[3], sw.bias[3]);
     __syncthreads();
@@ -183,10 +192,18 @@ int poisson_query_launch(const FaceQueryArgs& a, const SelNetDev& sel, const Fac
 {
     if (a.B <= 0) return 0;
     if (a.full) return set_error(-2, "poisson_query_launch: sample mode only");
-    if (threads == 512)
-        hipLaunchKernelGGL(poisson_query_kernel<512>, dim3(a.B), dim3(512), 0, stream, a, sel, pred);
-    else
-        hipLaunchKernelGGL(poisson_query_kernel<256>, dim3(a.B), dim3(256), 0, stream, a, sel, pred);
+    if (threads != 256 && threads != 512) return set_error(-2, "poisson_query_launch: threads must be 256 or 512");
+    if (a.feat_in) {
+        if (threads == 512)
+            hipLaunchKernelGGL((poisson_query_kernel<512, true>), dim3(a.B), dim3(512), 0, stream, a, sel, pred);
+        else
+            hipLaunchKernelGGL((poisson_query_kernel<256, true>), dim3(a.B), dim3(256), 0, stream, a, sel, pred);
+    } else {
+        if (threads == 512)
+            hipLaunchKernelGGL((poisson_query_kernel<512, false>), dim3(a.B), dim3(512), 0, stream, a, sel, pred);
+        else
+            hipLaunchKernelGGL((poisson_query_kernel<256, false>), dim3(a.B), dim3(256), 0, stream, a, sel, pred);
+    }
     DRW_HIP(hipGetLastError());
     return 0;
 }

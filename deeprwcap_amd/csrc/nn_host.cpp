@@ -458,6 +458,16 @@ int query_nn_launch(const NetWeights* w, const QueryArgs& q, uint8_t** scratch, 
         return timed(7, [&] { return poisson_query_launch(fa, w->sel, w->pred, fused, stream); });
     }
     if ((rc = timed(0, [&] { return selector_backbone_launch(grad, ba, w->sel, stream); }))) return rc;
+    if (!grad && !q.full) {
+        // Poisson sample mode with the backbone as its own kernel: head, face
+        // draw and predictor in one launch (the predictor's launch shape)
+        FaceQueryArgs fa{};
+        fa.B = B; fa.layers = q.layers; fa.cuboids = q.cuboids; fa.out = q.out; fa.kern_out = q.kern_out;
+        fa.seed = q.seed; fa.counter = q.counter; fa.qid = q.qid; fa.ctr_dev = q.ctr_dev;
+        fa.fb_list = q.fb_list; fa.fb_count = q.fb_count; fa.fb_cap = q.fb_cap; fa.fb_stats = q.fb_stats;
+        fa.probs_out = q.probs_out; fa.face_out = q.face_out; fa.feat_in = feat;
+        return timed(2, [&] { return poisson_query_launch(fa, w->sel, w->pred, launch_shape(kShapeFacePoisson, B), stream); });
+    }
     if ((rc = timed(1, [&] { return selector_head_launch(grad, ha, w->sel, stream); }))) return rc;
     FaceQueryArgs fa{};
     fa.B = B; fa.full = q.full; fa.layers = q.layers; fa.cuboids = q.cuboids; fa.face = face;

@@ -142,7 +142,8 @@ int drw_solver_stage_times(drw_solver* s, double* total_ms, int64_t* counts, int
  * kernels (one workgroup per query and face, 256 or 512 threads).
  * DRW_SHAPE_POISSON_FUSED is the Poisson query in one kernel (sample mode:
  * selector, head, face draw and predictor in one workgroup per query, 256 or
- * 512 threads), or 1: the separate selector, head and predictor kernels;
+ * 512 threads), or 1: the selector backbone as its own kernel, then head,
+ * face draw and predictor in one kernel (DRW_SHAPE_FACE_POISSON's threads);
  * both give bit-identical results. */
 enum drw_shape {
     DRW_SHAPE_FACE_POISSON = 0,
