@@ -550,18 +550,22 @@ __device__ __forceinline__ void face_solver_plus(FaceSmem<32>& s, const FaceNetD
     dw_conv<NTH, 32, 1, 32, 32>(s.X, s.Y, w.dw_w[0], w.dw_b[0], s.zero, s.gtab); __syncthreads();
     pw_mfma<NTH, 32, 32, 32, 32, 32>(s.Y, s.Y, w.frag[1], w.frag_lo[1], w.pw_bias[1], s.gtab); __syncthreads();
     dw_conv<NTH, 32, 1, 32, 32>(s.Y, s.X, w.dw_w[1], w.dw_b[1], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 32, 24, 32, 32, 32>(s.X, s.X, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
-    dw_conv<NTH, 24, 1, 32, 32>(s.X, s.Y, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 32, 24, 32, 32, 32>(s.Y, s.Y, w.frag[3], w.frag_lo[3], w.pw_bias[3], s.gtab); __syncthreads();
-    dw_conv<NTH, 24, 1, 32, 32>(s.Y, s.X, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
-    // from 16 channels on the activations use a 16-channel pixel stride (the
-    // first half of each buffer): the depthwise reads of consecutive pixels
-    // then fall on distinct banks, where the 32-channel stride aliases them
-    pw_mfma<NTH, 32, 16, 16, 32, 16>(s.X, s.Y, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
-    dw_conv<NTH, 16, 2, 16, 16>(s.Y, s.X, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 16, 16, 16, 16, 16>(s.X, s.X, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
-    dw_conv<NTH, 16, 2, 16, 16>(s.X, s.Y, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
-    pw_mfma<NTH, 16, 8, 16, 16, 16>(s.Y, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
+    // from 24 channels on the activations use a pixel stride of their channel
+    // count (24, then 16 halves; the front of each buffer): the 16 rows of an
+    // MFMA tile and the depthwise reads of consecutive pixels then spread
+    // over the banks, where the 32-channel stride aliases them.  A 24-channel
+    // row read with the layer's 32-wide K reads 8 halves of the next row for
+    // the K padding, whose weights are zero; every half the layers read
+    // there was written by an earlier layer, so the products are exact zeros.
+    pw_mfma<NTH, 32, 24, 32, 32, 24>(s.X, s.Y, w.frag[2], w.frag_lo[2], w.pw_bias[2], s.gtab); __syncthreads();
+    dw_conv<NTH, 24, 1, 24, 24>(s.Y, s.X, w.dw_w[2], w.dw_b[2], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 32, 24, 32, 24, 24>(s.X, s.X, w.frag[3], w.frag_lo[3], w.pw_bias[3], s.gtab); __syncthreads();
+    dw_conv<NTH, 24, 1, 24, 24>(s.X, s.Y, w.dw_w[3], w.dw_b[3], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 32, 16, 16, 24, 16>(s.Y, s.X, w.frag[4], w.frag_lo[4], w.pw_bias[4], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 2, 16, 16>(s.X, s.Y, w.dw_w[4], w.dw_b[4], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 16, 16, 16, 16>(s.Y, s.Y, w.frag[5], w.frag_lo[5], w.pw_bias[5], s.gtab); __syncthreads();
+    dw_conv<NTH, 16, 2, 16, 16>(s.Y, s.X, w.dw_w[5], w.dw_b[5], s.zero, s.gtab); __syncthreads();
+    pw_mfma<NTH, 16, 8, 16, 16, 16>(s.X, s.X, w.frag[6], w.frag_lo[6], w.pw_bias[6], s.gtab); __syncthreads();
     float* fX = (float*)s.X;
     float* fY = (float*)s.Y;
     dw_conv<NTH, 8, 3, 16, 8>(s.X, fY, w.dw_w[6], w.dw_b[6], s.zero, s.gtab); __syncthreads();
